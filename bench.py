@@ -1,0 +1,369 @@
+"""Benchmark of the HIP stacked solve on one MI355X.
+
+Default workload (the headline): mixed-precision refined solve of a stack of
+64 systems of order 2048 with 8 right-hand sides each, cond_2(A) = 1e4:
+A = U diag(s) V^T with U, V Haar orthogonal (QR of standard-normal matrices,
+sign-fixed), s = logspace(0, -4, 2048); B ~ N(0, 1); PCG64 seed 3. The 64
+matrices pair 8 Haar U with 8 Haar V (A_{8i+j} = U_i S V_j^T), which keeps the
+host-side QR cost to 16 factorisations; every matrix has cond_2 = 1e4 exactly.
+
+A step is one call of the refined solver on the whole stack (inputs resident
+in HBM): low-precision LU, FP64 refinement to backward error <= 1e-12 for every
+right-hand side, with the backward error reported per column. The same step
+in plain FP64 (LU + getrs + backward-error report) is timed beside it.
+`value` = systems solved per second by the faster mixed-precision mode.
+
+Other workloads (--workload): gemm (8192^3 FP64 DGEMM, TFLOP/s), plain64
+(256 x 64 x 64 FP64 stacked solve, solves/s).
+
+Prints ONE JSON line. --gpus above 1 exits: this build runs on one GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3, "bf16": 2500.0}
+PROF_KINDS = {0: ("gemm_f64", "mfma", "f64"), 1: ("gemm_f32", "mfma", "f32"), 2: ("gemm_bf16split", "mfma", "bf16"),
+              3: ("lu_panel", "hbm", None), 4: ("move_trsm", "hbm", None), 5: ("getrs", "hbm", None),
+              6: ("residual_f64", "hbm", None), 7: ("lu_wave64", "hbm", None), 8: ("vbatch", "hbm", None)}
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_info():
+    model = platform.processor() or ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return model, os.cpu_count()
+
+
+def pin_one_core():
+    try:
+        cpus = sorted(os.sched_getaffinity(0))
+        os.sched_setaffinity(0, {cpus[0]})
+        return cpus
+    except (AttributeError, OSError):
+        return None
+
+
+def unpin(cpus):
+    if cpus:
+        os.sched_setaffinity(0, set(cpus))
+
+
+def haar(rng, n):
+    q, r = np.linalg.qr(rng.standard_normal((n, n)))
+    return q * np.sign(np.diag(r))
+
+
+def prof_report(lib, ctypes):
+    out = {}
+    for kind, (name, bound, dt) in PROF_KINDS.items():
+        ms, fl, by, nl = ctypes.c_double(), ctypes.c_double(), ctypes.c_double(), ctypes.c_int64()
+        lib.npph_prof_get(kind, ctypes.byref(ms), ctypes.byref(fl), ctypes.byref(by), ctypes.byref(nl))
+        if nl.value:
+            out[name] = {"ms": ms.value, "flops": fl.value, "bytes": by.value, "launches": nl.value,
+                         "bound": bound, "dtype": dt}
+    return out
+
+
+def roofline_from(prof, kind_name, extra=None):
+    p = prof[kind_name]
+    avg_s = p["ms"] / p["launches"] / 1e3
+    if p["bound"] == "mfma":
+        achieved = p["flops"] / p["launches"] / avg_s / 1e12
+        peak = PEAK_TFLOPS[p["dtype"]]
+        r = {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
+             "traffic": None}
+    else:
+        achieved = p["bytes"] / p["launches"] / avg_s / 1e9
+        r = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+             "frac": achieved / HBM_PEAK_GBS, "traffic": None}
+    r["kernel"] = kind_name
+    r["avg_launch_us"] = avg_s * 1e6
+    r["launches_per_step"] = p["launches"]
+    if extra:
+        r.update(extra)
+    return r
+
+
+def bench_mixed(args):
+    import ctypes
+
+    import numpp_amd
+    import oracle
+    from numpp_amd import _lib
+    from numpp_amd import linalg as nla
+    from numpp_amd.device import DeviceArray, Timer
+
+    lib = numpp_amd.require_device()
+    count, n, nrhs, cond = args.count, args.n, 8, 1e4
+    rng = np.random.default_rng(3)
+    t0 = time.time()
+    npair = int(np.ceil(np.sqrt(count)))
+    s = np.logspace(0.0, -np.log10(cond), n)
+    Us = np.stack([haar(rng, n) * s for _ in range(npair)])
+    Vt = np.stack([np.ascontiguousarray(haar(rng, n).T) for _ in range(npair)])
+    B = rng.standard_normal((count, n, nrhs))
+    log(f"inputs: {2 * npair} Haar factors of order {n} in {time.time() - t0:.1f}s")
+    dUs, dVt = DeviceArray.from_host(Us), DeviceArray.from_host(Vt)
+    dA = DeviceArray((count, n, n))
+    for m in range(count):
+        i, j = divmod(m, npair)
+        _lib.check(lib.npph_dgemm_dev(n, n, n, 1.0, dUs.ptr.value + i * n * n * 8, n, 0,
+                                      dVt.ptr.value + j * n * n * 8, n, 0, 0.0, dA.ptr.value + m * n * n * 8,
+                                      n, 0, 1, None))
+    dB = DeviceArray.from_host(B)
+    dX = DeviceArray((count, n, nrhs))
+    dE = DeviceArray((count, nrhs))
+    dI = DeviceArray((count,), np.int32)
+    dF = DeviceArray((count,), np.int32)
+    del Us, Vt, dUs, dVt
+
+    def run(prec, terms=6):
+        o = nla.refine_opts(prec, tol=1e-12, max_iter=10, fallback=True, bf16_terms=terms)
+        nla.gesv_refined_dev(dA, dB, dX, dE, dI, dF, o)
+
+    results = {}
+    modes = [("fp64", 0), ("fp32", 0), ("bf16", 6), ("bf16", 3)]
+    for prec, terms in modes:
+        key = prec if prec != "bf16" else f"bf16x{terms}"
+        for _ in range(args.warmup):
+            run(prec, terms)
+        lib.npph_device_sync()
+        tm = Timer()
+        lat = []
+        tm.start()
+        for _ in range(args.steps):
+            t1 = time.perf_counter()
+            run(prec, terms)
+            lib.npph_device_sync()
+            lat.append((time.perf_counter() - t1) * 1e3)
+        ms = tm.stop() / args.steps
+        x = dX.to_host()
+        results[key] = {"ms": ms, "solves_per_s": count / (ms / 1e3), "berr_max": float(dE.to_host().max()),
+                        "iters": dI.to_host().tolist(), "info_nonzero": int((dF.to_host() != 0).sum()),
+                        "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)), "x": x}
+        log(f"{key}: {ms:.3f} ms/step, {results[key]['solves_per_s']:.1f} solves/s, "
+            f"berr_max {results[key]['berr_max']:.2e}, iters {sorted(set(results[key]['iters']))}")
+    x64 = results["fp64"]["x"]
+    for key, r in results.items():
+        r["fwd_err_vs_fp64_max"] = float(np.max(np.abs(r["x"] - x64).max(axis=(1, 2)) / np.abs(x64).max(axis=(1, 2))))
+    mixed = [k for k in results if k != "fp64" and results[k]["berr_max"] <= 1e-12 and
+             all(i >= 0 for i in results[k]["iters"])]
+    best = min(mixed, key=lambda k: results[k]["ms"]) if mixed else "fp64"
+
+    # Profiled pass (one extra step per mode with per-kernel HIP events).
+    prof = {}
+    for key in (best, "fp64"):
+        prec = "bf16" if key.startswith("bf16") else key
+        terms = int(key[5:]) if key.startswith("bf16") else 0
+        lib.npph_prof_reset()
+        lib.npph_prof_enable(1)
+        run(prec, terms)
+        lib.npph_device_sync()
+        lib.npph_prof_enable(0)
+        prof[key] = prof_report(lib, ctypes)
+    lib.npph_prof_reset()
+    dom = max(prof[best], key=lambda k: prof[best][k]["ms"])
+    roof = roofline_from(prof[best], dom)
+    step_ms = results[best]["ms"]
+    roof["share_of_step"] = prof[best][dom]["ms"] / step_ms
+
+    # CPU baseline: the oracle's FP64 LU + solve (NumPP's portable kernel,
+    # restated) on one core for a bounded sample of the same workload.
+    cpu = None
+    if not args.no_cpu:
+        A0 = dA.to_host()[: args.cpu_sample]
+        cpus = pin_one_core()
+        t1 = time.perf_counter()
+        xo = [oracle.solve(A0[i], B[i]) for i in range(args.cpu_sample)]
+        dt = time.perf_counter() - t1
+        unpin(cpus)
+        model, ncpu = cpu_info()
+        cpu = {"value": args.cpu_sample / dt, "unit": "solves/s", "cores": 1, "kind": "port",
+               "sample": f"{args.cpu_sample} of {count} systems (n={n}, 8 RHS), FP64 LU+solve, "
+                         f"oracle C restatement -O3 -ffp-contract=off, 1 of {ncpu} cores ({model})",
+               "seconds": dt}
+        fe = max(float(np.abs(x64[i] - xo[i]).max() / np.abs(xo[i]).max()) for i in range(args.cpu_sample))
+        results["fp64"]["fwd_err_vs_oracle_sample"] = fe
+        log(f"cpu baseline: {dt:.2f}s for {args.cpu_sample} systems; fp64 gpu vs oracle fwd err {fe:.2e}")
+
+    r = results[best]
+    line = {
+        "metric": "solves/s + p99 latency, mixed-prec refined vs FP64 LU at backward err <=1e-12",
+        "value": r["solves_per_s"], "unit": "solves/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": r["ms"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if best == "fp64" else ("f32+f64" if best == "fp32" else "bf16+f64"),
+        "data": "synthetic",
+        "config": {"workload": f"refined stacked solve: {count} x {n}x{n} float64, {nrhs} RHS, cond 1e4, "
+                               f"Haar U/V pairs, seed 3", "count": count, "n": n, "nrhs": nrhs,
+                   "mode": best, "tol": 1e-12, "parallelism": "single-gpu"},
+        "p50_ms": r["p50_ms"], "p99_ms": r["p99_ms"],
+        "fp64_solves_per_s": results["fp64"]["solves_per_s"], "fp64_ms_per_step": results["fp64"]["ms"],
+        "speedup_vs_fp64": results["fp64"]["ms"] / r["ms"],
+        "berr_max": r["berr_max"], "fwd_err_vs_fp64_max": r["fwd_err_vs_fp64_max"],
+        "modes": {k: {kk: v for kk, v in rr.items() if kk not in ("x", "iters")} |
+                  {"iters_max": max(rr["iters"]), "iters_min": min(rr["iters"])} for k, rr in results.items()},
+        "roofline": roof,
+        "kernel_ms_per_step": {k: {kk: round(vv["ms"], 4) for kk, vv in v.items()} for k, v in prof.items()},
+        "cpu_baseline": cpu,
+    }
+    return line
+
+
+def bench_gemm(args):
+    import ctypes
+
+    import numpp_amd
+    import oracle
+    from numpp_amd import _lib
+    from numpp_amd.device import DeviceArray, Timer
+
+    lib = numpp_amd.require_device()
+    n = args.n if args.n != 2048 else 8192
+    rng = np.random.default_rng(1)
+    a = rng.standard_normal((n, n))
+    b = rng.standard_normal((n, n))
+    dA, dB, dC = DeviceArray.from_host(a), DeviceArray.from_host(b), DeviceArray((n, n))
+    del a, b
+
+    def run():
+        _lib.check(lib.npph_dgemm_dev(n, n, n, 1.0, dA.ptr, n, 0, dB.ptr, n, 0, 0.0, dC.ptr, n, 0, 1, None))
+
+    for _ in range(args.warmup):
+        run()
+    lib.npph_device_sync()
+    tm = Timer()
+    tm.start()
+    for _ in range(args.steps):
+        run()
+    ms = tm.stop() / args.steps
+    flops = 2.0 * n ** 3
+    lib.npph_prof_reset()
+    lib.npph_prof_enable(1)
+    run()
+    lib.npph_device_sync()
+    lib.npph_prof_enable(0)
+    prof = prof_report(lib, ctypes)
+    roof = roofline_from(prof, "gemm_f64")
+    cpu = None
+    if not args.no_cpu:
+        m = 2048
+        r2 = np.random.default_rng(1)
+        a2, b2 = r2.standard_normal((m, m)), r2.standard_normal((m, m))
+        cpus = pin_one_core()
+        t1 = time.perf_counter()
+        oracle.matmul(a2, b2)
+        dt = time.perf_counter() - t1
+        unpin(cpus)
+        model, ncpu = cpu_info()
+        cpu = {"value": 2.0 * m ** 3 / dt / 1e12, "unit": "TFLOP/s", "cores": 1, "kind": "port",
+               "sample": f"2048^3 FP64 GEMM, oracle i-p-j loop -O3 -ffp-contract=off, 1 of {ncpu} cores ({model})",
+               "seconds": dt}
+    return {"metric": "FP64 DGEMM TFLOP/s (8192^3, device-resident)", "value": flops / (ms / 1e3) / 1e12,
+            "unit": "TFLOP/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": f"C = A B, {n}x{n} float64, N(0,1) seed 1", "parallelism": "single-gpu"},
+            "roofline": roof, "cpu_baseline": cpu}
+
+
+def bench_plain64(args):
+    import ctypes
+
+    import numpp_amd
+    import oracle
+    from numpp_amd import _lib
+    from numpp_amd.device import DeviceArray, Timer
+
+    lib = numpp_amd.require_device()
+    rng = np.random.default_rng(0)
+    a = rng.uniform(-1.0, 1.0, (256, 64, 64)) + 64.0 * np.eye(64)
+    b = rng.standard_normal((256, 64))
+    dA0, dB0 = DeviceArray.from_host(a), DeviceArray.from_host(b)
+    dA, dB = DeviceArray((256, 64, 64)), DeviceArray((256, 64))
+
+    def run():
+        dA.copy_from_device(dA0)
+        dB.copy_from_device(dB0)
+        _lib.check(lib.npph_gesv_batched_dev(256, 64, 1, dA.ptr, dB.ptr, None, None, None))
+
+    for _ in range(args.warmup):
+        run()
+    lib.npph_device_sync()
+    tm = Timer()
+    tm.start()
+    for _ in range(args.steps):
+        run()
+    ms = tm.stop() / args.steps
+    lib.npph_prof_reset()
+    lib.npph_prof_enable(1)
+    run()
+    lib.npph_device_sync()
+    lib.npph_prof_enable(0)
+    prof = prof_report(lib, ctypes)
+    roof = roofline_from(prof, "lu_wave64")
+    x = dB.to_host()
+    xo, _ = oracle.batched_solve(a, b)
+    cpus = pin_one_core()
+    t1 = time.perf_counter()
+    reps = 20
+    for _ in range(reps):
+        oracle.batched_solve(a, b)
+    dt = (time.perf_counter() - t1) / reps
+    unpin(cpus)
+    model, ncpu = cpu_info()
+    return {"metric": "FP64 stacked solve solves/s (256 x 64x64, 1 RHS)", "value": 256 / (ms / 1e3),
+            "unit": "solves/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": "256 x 64x64 float64, U[-1,1)+64I, 1 RHS, seed 0", "parallelism": "single-gpu"},
+            "rel_err_vs_oracle": float(np.abs(x - xo).max() / np.abs(xo).max()),
+            "roofline": roof,
+            "cpu_baseline": {"value": 256 / dt, "unit": "solves/s", "cores": 1, "kind": "port",
+                             "sample": f"full 256-system batch, oracle, 1 of {ncpu} cores ({model})"}}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="mixed", choices=["mixed", "gemm", "plain64"])
+    ap.add_argument("--count", type=int, default=64)
+    ap.add_argument("--n", type=int, default=2048)
+    ap.add_argument("--cpu-sample", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the stacked solve shards by "
+              f"system across GPUs with no exchange, which this build does not launch.", file=sys.stderr)
+        sys.exit(2)
+    if os.environ.get("WORLD_SIZE", "1") not in ("", "1"):
+        print("bench.py: multi-rank launch is not supported by this single-GPU build", file=sys.stderr)
+        sys.exit(2)
+    fn = {"mixed": bench_mixed, "gemm": bench_gemm, "plain64": bench_plain64}[args.workload]
+    line = fn(args)
+    print(json.dumps(line, default=float))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,181 @@
+/*
+ * numpp_hip.h — C ABI of the MI355X (gfx950) device backend for NumPP's
+ * numpp::matmul and stacked numpp::linalg::solve.
+ *
+ * Plain C: POD arguments, int status codes, no C++ or torch types. Two call
+ * families:
+ *   - host-pointer calls (npph_gemm, npph_gesv_batched, ...) follow the
+ *     reference GpuVTable slot contract (include/numpp/backend/gpu_vtable.hpp:
+ *     14-16, 22-26): contiguous row-major host buffers in the native dtype,
+ *     synchronous, results in the host buffers on return;
+ *   - device-resident calls (suffix _dev) take device pointers and a hipStream_t
+ *     passed as void* (NULL = the library's own stream) and are asynchronous.
+ *
+ * Status: NPPH_OK (0) or a negative NPPH_ERR_*; npph_last_error() returns a
+ * thread-local message for the last failure on the calling thread.
+ * Row-major everywhere (NumPP is C-order). Pivots: ipiv[k] is the 0-based row
+ * position swapped with row k at step k (LAPACK convention, 0-based); the
+ * final permutation of linalg.cpp:58-62 follows by applying the swaps in order.
+ * Singularity uses NumPP's exact-zero rule (linalg.cpp:57): info = k+1 when the
+ * largest candidate pivot at step k is exactly 0.0, else 0.
+ */
+#ifndef NUMPP_HIP_H
+#define NUMPP_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NPPH_OK 0
+#define NPPH_ERR_NO_DEVICE (-1)
+#define NPPH_ERR_INVALID (-2)
+#define NPPH_ERR_HIP (-3)
+#define NPPH_ERR_OOM (-4)
+#define NPPH_ERR_UNSUPPORTED (-5)
+#define NPPH_ERR_BUDGET (-6)
+#define NPPH_ERR_NOT_FOUND (-7)
+
+/* dtype ids: the values of numpp::DTypeId (include/numpp/core/dtype.hpp:16-24). */
+#define NPPH_DT_F32 10
+#define NPPH_DT_F64 11
+
+/* Factorisation precision for the refined solver. */
+#define NPPH_PREC_FP64 0    /* plain FP64 LU, no refinement needed */
+#define NPPH_PREC_FP32 1    /* FP32 LU on MFMA (FP32 accumulate) + FP64 refinement */
+#define NPPH_PREC_BF16 2    /* BF16-split MFMA updates with per-panel scaling + FP64 refinement */
+
+/* ---------------- runtime ---------------- */
+/* Probes the device once (gfx950 check), creates the library stream.
+ * No reference counterpart: the reference probes lazily on every call
+ * (src/backend/cuda_backend.cu:120-123). */
+int npph_init(void);
+/* 1 when a usable gfx950 device is present (mirrors the non-null accessor
+ * contract of gpu_vtable(), cuda_backend.cu:320), else 0. */
+int npph_available(void);
+const char* npph_last_error(void);
+const char* npph_device_name(void);
+int npph_cu_count(void);
+
+/* Device memory / stream plumbing for callers that keep data resident. */
+int npph_malloc(void** ptr, uint64_t bytes);
+int npph_free(void* ptr);
+int npph_memcpy_htod(void* dst, const void* src, uint64_t bytes, void* stream);
+int npph_memcpy_dtoh(void* dst, const void* src, uint64_t bytes, void* stream);
+int npph_memcpy_dtod(void* dst, const void* src, uint64_t bytes, void* stream);
+int npph_memset(void* dst, int value, uint64_t bytes, void* stream);
+int npph_stream_create(void** stream);
+int npph_stream_destroy(void* stream);
+int npph_stream_sync(void* stream);
+int npph_device_sync(void);
+int npph_event_create(void** ev);
+int npph_event_destroy(void* ev);
+int npph_event_record(void* ev, void* stream);
+int npph_event_elapsed_ms(void* start, void* stop, float* ms);
+
+/* ---------------- GEMM ---------------- */
+/* Replaces GpuVTable::gemm (gpu_vtable.hpp:22-26; CUDA provider gemm_impl/
+ * gemm_t, src/backend/cuda_backend.cu:189-206, 288-293): row-major
+ * C[m,n] = A[m,k] B[k,n] on host buffers, dtype F32 or F64, synchronous.
+ * Returns NPPH_ERR_UNSUPPORTED for other dtypes (the vtable slot then returns
+ * false and matmul falls back to the CPU, backend.cpp:144-152). */
+int npph_gemm(int dtype, int64_t m, int64_t n, int64_t k, const void* A, const void* B, void* C);
+
+/* Device-resident row-major C = alpha A B + beta C (FP64 MFMA), batched over
+ * `batch` matrices with element strides sA/sB/sC. No reference counterpart
+ * (the reference keeps no device-resident arrays, docs/gpu-backends.md:30-31);
+ * it is the roofline path of the matmul slot above. */
+int npph_dgemm_dev(int64_t m, int64_t n, int64_t k, double alpha, const double* A, int64_t lda,
+                   int64_t sA, const double* B, int64_t ldb, int64_t sB, double beta, double* C,
+                   int64_t ldc, int64_t sC, int64_t batch, void* stream);
+/* The same in FP32 (FP32 MFMA, FP32 accumulation). */
+int npph_sgemm_dev(int64_t m, int64_t n, int64_t k, float alpha, const float* A, int64_t lda,
+                   int64_t sA, const float* B, int64_t ldb, int64_t sB, float beta, float* C,
+                   int64_t ldc, int64_t sC, int64_t batch, void* stream);
+
+/* Device-resident C -= A B for FP32 matrices through the split-BF16 MFMA
+ * update of the low-precision LU (terms 1, 3 or 6; K <= 64). No reference
+ * counterpart; exposed so the update can be checked on its own. */
+int npph_bf16split_update_dev(int64_t M, int64_t N, int64_t K, const float* A, int64_t lda,
+                              const float* B, int64_t ldb, float* C, int64_t ldc, int terms,
+                              void* stream);
+
+/* ---------------- stacked solve (plain) ---------------- */
+/* New GpuVTable slot gesv_batched: replaces the per-matrix loop of
+ * batched::solve (src/linalg/batched.cpp:90-100) and solve_impl<double>
+ * (src/linalg/linalg.cpp:92-103; lu_decompose 48-71, lu_solve 73-90) with one
+ * device call for the whole stack. A: count x n x n, overwritten with the packed
+ * LU factors (L unit-lower below the diagonal, U on and above). B: count x n x
+ * nrhs, overwritten with X. ipiv (nullable): count x n. info (nullable): count.
+ * dtype F64 (F32 is computed in F64 and rounded back, as linalg.cpp:20-26).
+ * Returns NPPH_OK even when systems are singular; check info. */
+int npph_gesv_batched(int dtype, int64_t count, int64_t n, int64_t nrhs, void* A, void* B,
+                      int32_t* ipiv, int32_t* info);
+
+/* Device-resident FP64 LU with partial pivoting of a uniform stack (element
+ * stride strideA between matrices), getrs-style solve with those factors, and
+ * both together. Same factor / pivot / info conventions as above. */
+int npph_getrf_batched_dev(int64_t count, int64_t n, double* A, int64_t strideA, int32_t* ipiv,
+                           int32_t* info, void* stream);
+int npph_getrs_batched_dev(int64_t count, int64_t n, int64_t nrhs, const double* LU,
+                           int64_t strideA, const int32_t* ipiv, double* B, int64_t strideB,
+                           void* stream);
+int npph_gesv_batched_dev(int64_t count, int64_t n, int64_t nrhs, double* A, double* B,
+                          int32_t* ipiv, int32_t* info, void* stream);
+
+/* ---------------- mixed precision + FP64 iterative refinement ---------------- */
+/* Options for the refined solver. No reference counterpart (NumPP always
+ * factors in FP64, linalg.cpp:20-26); this is the precision addition. */
+typedef struct npph_refine_opts {
+  int precision;      /* NPPH_PREC_* */
+  int max_iter;       /* refinement sweeps before FP64 fallback (e.g. 10) */
+  double tol;         /* target normwise backward error, e.g. 1e-12 */
+  int fallback_fp64;  /* 1: systems not converged after max_iter are re-solved in plain FP64 */
+  int bf16_terms;     /* BF16 split products per update: 1, 3 (2-term split) or 6 (3-term) */
+} npph_refine_opts;
+
+/* Solves a uniform stack A X = B (A count x n x n, B/X count x n x nrhs, FP64,
+ * device pointers; A and B are not modified). Factorises in opts->precision,
+ * refines in FP64 on the device (r = b - A x, correction with the
+ * low-precision factors, x += d) until every RHS column meets
+ * eta = ||r||_inf / (||A||_inf ||x||_inf + ||b||_inf) <= tol.
+ * Outputs (device, nullable): berr[count*nrhs] final eta per column,
+ * iters[count] sweeps used (-1 = fell back to FP64), info[count] singular step
+ * + 1 or 0. */
+int npph_gesv_refined_batched_dev(const npph_refine_opts* opts, int64_t count, int64_t n,
+                                  int64_t nrhs, const double* A, const double* B, double* X,
+                                  double* berr, int32_t* iters, int32_t* info, void* stream);
+
+/* ---------------- profiling ---------------- */
+/* Per-kernel-class timing with HIP events around every launch (off by default;
+ * no reference counterpart: NumPP has no timers or benchmarks).
+ * Kinds: 0 FP64 GEMM, 1 FP32 GEMM, 2 split-BF16 update, 3 LU panel,
+ * 4 row-move + trsm, 5 getrs, 6 FP64 residual, 7 one-wave LU, 8 uneven batch.
+ * npph_prof_get synchronises and returns summed milliseconds, algorithmic
+ * flops and bytes, and the launch count of one kind since the last reset. */
+#define NPPH_PROF_GEMM_F64 0
+#define NPPH_PROF_GEMM_F32 1
+#define NPPH_PROF_GEMM_BF16 2
+#define NPPH_PROF_PANEL 3
+#define NPPH_PROF_MOVE_TRSM 4
+#define NPPH_PROF_GETRS 5
+#define NPPH_PROF_RESIDUAL 6
+#define NPPH_PROF_SMALL_LU 7
+#define NPPH_PROF_VBATCH 8
+int npph_prof_enable(int on);
+int npph_prof_reset(void);
+int npph_prof_get(int kind, double* ms, double* flops, double* bytes, int64_t* launches);
+
+/* ---------------- drop-in vtable ---------------- */
+/* Address of the numpp::GpuVTable provider struct (layout of
+ * include/numpp_hip/gpu_vtable.hpp, i.e. the reference gpu_vtable.hpp:17-53
+ * plus appended solve slots), or NULL when no usable device is present —
+ * the same contract as numpp::gpu_vtable() (gpu_vtable.hpp:56). */
+const void* npph_gpu_vtable(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NUMPP_HIP_H */

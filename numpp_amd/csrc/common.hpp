@@ -1,0 +1,94 @@
+// Shared definitions for the numpp HIP backend (gfx950 / MI355X).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstddef>
+
+#include "numpp_hip.h"
+
+namespace npph {
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef short v8s __attribute__((ext_vector_type(8)));
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+
+constexpr int kWave = 64;
+
+// ---- error reporting (thread-local message, C-ABI status codes) ----
+void set_error(const char* fmt, ...);
+int hip_fail(hipError_t e, const char* what, const char* file, int line);
+
+#define NPPH_HIP(x)                                                        \
+  do {                                                                     \
+    hipError_t e_ = (x);                                                   \
+    if (e_ != hipSuccess) return ::npph::hip_fail(e_, #x, __FILE__, __LINE__); \
+  } while (0)
+
+#define NPPH_TRY(x)               \
+  do {                            \
+    int s_ = (x);                 \
+    if (s_ != NPPH_OK) return s_; \
+  } while (0)
+
+// ---- device state ----
+// One process-wide context per device, created once by ensure_init(): device
+// probe (gfx950 check), a default stream, and a grow-only workspace arena.
+struct Workspace {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+};
+
+int ensure_init();                // NPPH_OK or NPPH_ERR_NO_DEVICE
+hipStream_t default_stream();     // library-owned stream for host-pointer calls
+int cu_count();                   // compute units on the device
+// Returns a device buffer of at least `bytes` from a named workspace slot
+// (grow-only, freed at process exit). Slots let concurrent phases of one call
+// hold separate buffers; callers that share a slot must not overlap.
+int workspace(int slot, size_t bytes, void** out);
+enum WsSlot { kWsA = 0, kWsB, kWsPiv, kWsInfo, kWsLo, kWsR, kWsX, kWsNorm, kWsMisc, kWsMove, kWsPerm, kWsD, kWsRlo, kWsCount };
+
+// Grow-only helper arena owned by the caller (used by the serving loop and the
+// uneven-batch plan so their buffers do not alias the per-call slots).
+inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+// ---- wave-level helpers ----
+__device__ __forceinline__ double shfl_xor_d(double v, int m) { return __shfl_xor(v, m, kWave); }
+__device__ __forceinline__ int shfl_xor_i(int v, int m) { return __shfl_xor(v, m, kWave); }
+
+__device__ __forceinline__ double readlane_d(double v, int lane) {
+  int2 u = __builtin_bit_cast(int2, v);
+  u.x = __builtin_amdgcn_readlane(u.x, lane);
+  u.y = __builtin_amdgcn_readlane(u.y, lane);
+  return __builtin_bit_cast(double, u);
+}
+__device__ __forceinline__ float readlane_f(float v, int lane) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), lane));
+}
+template <class T>
+__device__ __forceinline__ T readlane(T v, int lane);
+template <>
+__device__ __forceinline__ double readlane<double>(double v, int lane) { return readlane_d(v, lane); }
+template <>
+__device__ __forceinline__ float readlane<float>(float v, int lane) { return readlane_f(v, lane); }
+
+// Pivot choice with NumPP's rule (linalg.cpp:55-56): the largest |value|, and
+// among equal magnitudes the one at the smallest current row position. Returns
+// true when (v1, p1) beats (v0, p0).
+__device__ __forceinline__ bool piv_better(double v1, int p1, double v0, int p0) {
+  return v1 > v0 || (v1 == v0 && p1 < p0);
+}
+
+// Wave-wide argmax of (v, pos); every lane receives the winner.
+__device__ __forceinline__ void wave_argmax(double& v, int& pos) {
+#pragma unroll
+  for (int m = 1; m < kWave; m <<= 1) {
+    double ov = shfl_xor_d(v, m);
+    int op = shfl_xor_i(pos, m);
+    if (piv_better(ov, op, v, pos)) { v = ov; pos = op; }
+  }
+}
+
+}  // namespace npph

@@ -1,0 +1,37 @@
+// Internal GEMM launcher shared by matmul and the blocked LU trailing update.
+#pragma once
+
+#include <cstring>
+
+#include "common.hpp"
+
+namespace npph {
+
+// Row-major C = alpha * A(MxK) * B(KxN) + beta * C over `batch` matrices with
+// element strides sA / sB / sC between consecutive matrices.
+template <class T>
+struct GemmArgs {
+  int64_t M, N, K;
+  T alpha, beta;
+  const T* A;
+  int64_t lda, sA;
+  const T* B;
+  int64_t ldb, sB;
+  T* C;
+  int64_t ldc, sC;
+  int tiles_m, tiles_n, vec;  // filled by gemm_launch
+};
+
+template <class T>
+int gemm_launch(GemmArgs<T> p, int64_t batch, hipStream_t s);
+
+// Split-BF16 update C -= A * B for FP32 matrices (the low-precision LU's
+// trailing update): each operand is split into `terms`-1 BF16 pieces after a
+// power-of-two scaling per panel and the partial products are accumulated in
+// FP32 on the BF16 MFMA. terms: 1 (plain BF16), 3 (two-piece split, 3
+// products) or 6 (three-piece split, 6 products).
+int gemm_bf16split_update(int64_t M, int64_t N, int64_t K, const float* A, int64_t lda, int64_t sA,
+                          const float* B, int64_t ldb, int64_t sB, float* C, int64_t ldc,
+                          int64_t sC, int64_t batch, int terms, hipStream_t s);
+
+}  // namespace npph

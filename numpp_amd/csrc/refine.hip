@@ -1,0 +1,426 @@
+// Mixed-precision stacked solve with FP64 iterative refinement.
+//
+// The LU factorisation runs in FP32 (FP32 MFMA trailing updates) or with
+// split-BF16 MFMA trailing updates; FP64 is used on the device only for the
+// residual r = b - A x, the solution update x += d and the backward-error test
+//   eta_j = ||r_j||_inf / (||A||_inf ||x_j||_inf + ||b_j||_inf)
+// per right-hand-side column. Each sweep: residual (reads A once in FP64, the
+// HBM-bound kernel), eta + convergence per system, power-of-two scaling of r
+// into FP32, getrs with the low-precision factors, x += d. A system leaves the
+// loop when every column meets tol; one that stagnates (eta not halving) or
+// reaches max_iter is re-solved with the plain FP64 LU when fallback is on.
+// Per-system arithmetic never depends on the other systems in the stack.
+#include <vector>
+
+#include "common.hpp"
+#include "lu.hpp"
+#include "prof.hpp"
+
+namespace npph {
+
+int gesv_dev_impl(int64_t count, int n, int nrhs, double* A, int64_t strideA, double* B, int64_t strideB,
+                  int32_t* ipiv, int32_t* info, hipStream_t s);
+
+namespace {
+
+__device__ __forceinline__ void atomic_max_nonneg(double* addr, double v) {
+  atomicMax(reinterpret_cast<unsigned long long*>(addr), (unsigned long long)__double_as_longlong(v));
+}
+
+// A (FP64) -> Alo (FP32) and ||A||_inf per matrix. One wave per row.
+__global__ __launch_bounds__(256) void demote_kernel(int64_t count, int n, const double* A, float* Alo,
+                                                     double* anorm) {
+  const int64_t gw = ((int64_t)blockIdx.x * 256 + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (gw >= count * n) return;
+  const int64_t mat = gw / n;
+  const double* a = A + gw * n;
+  float* o = Alo + gw * n;
+  double s = 0.0;
+  for (int j = lane; j < n; j += 64) {
+    const double v = a[j];
+    o[j] = (float)v;
+    s += fabs(v);
+  }
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) s += __shfl_xor(s, m, 64);
+  if (lane == 0) atomic_max_nonneg(anorm + mat, s);
+}
+
+// ||b_j||_inf per (matrix, column) and the power-of-two scale for sweep 0.
+__global__ void bnorm_kernel(int64_t count, int n, int nrhs, const double* B, double* bmax) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count * n * nrhs) return;
+  const int64_t mat = i / ((int64_t)n * nrhs);
+  const int c = (int)(i % nrhs);
+  atomic_max_nonneg(bmax + mat * nrhs + c, fabs(B[i]));
+}
+
+// ||A||_inf per matrix only (plain FP64 path, for the reported eta).
+__global__ __launch_bounds__(256) void rowabs_kernel(int64_t count, int n, const double* A, double* anorm) {
+  const int64_t gw = ((int64_t)blockIdx.x * 256 + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (gw >= count * n) return;
+  const double* a = A + gw * n;
+  double s = 0.0;
+  for (int j = lane; j < n; j += 64) s += fabs(a[j]);
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) s += __shfl_xor(s, m, 64);
+  if (lane == 0) atomic_max_nonneg(anorm + gw / n, s);
+}
+
+__global__ void zero_int_kernel(int32_t* p, int n) {
+  if ((int)threadIdx.x < n) p[threadIdx.x] = 0;
+}
+
+__global__ void copy_masked_int_kernel(int64_t count, const int32_t* src, int32_t* dst, const int32_t* mask) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count && mask[i]) dst[i] = src[i];
+}
+
+__global__ void fill_kernel(double* p, int64_t n, double v) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__global__ void init_state_kernel(int64_t count, const int32_t* info, int32_t* active, int32_t* iters) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  active[i] = info[i] == 0 ? 1 : 2;  // a low-precision zero pivot goes to the FP64 fallback
+  iters[i] = 0;
+}
+
+__device__ __forceinline__ double pow2_scale(double m) {
+  // 2^-e with m in [2^e, 2^(e+1)); 1 for zero / non-finite.
+  if (!(m > 0.0) || !isfinite(m)) return 1.0;
+  return ldexp(1.0, -ilogb(m));
+}
+
+// Rlo = fp32(R * scale) (per column scale), only for active systems.
+__global__ void scale_demote_kernel(int64_t count, int n, int nrhs, const double* R, const double* scale,
+                                    float* Rlo, const int32_t* active) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t per = (int64_t)n * nrhs;
+  if (i >= count * per) return;
+  const int64_t mat = i / per;
+  if (active[mat] != 1) return;
+  const int c = (int)(i % nrhs);
+  Rlo[i] = (float)(R[i] * scale[mat * nrhs + c]);
+}
+
+// X (+)= D / scale for active systems; first sweep writes instead of adding.
+__global__ void update_kernel(int64_t count, int n, int nrhs, double* X, const float* D, const double* scale,
+                              const int32_t* active, int first) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t per = (int64_t)n * nrhs;
+  if (i >= count * per) return;
+  const int64_t mat = i / per;
+  if (active[mat] != 1) return;
+  const int c = (int)(i % nrhs);
+  const double d = (double)D[i] / scale[mat * nrhs + c];
+  X[i] = first ? d : X[i] + d;
+}
+
+__global__ void count_sweep_kernel(int64_t count, const int32_t* active, int32_t* iters) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count && active[i] == 1) iters[i] += 1;
+}
+
+// R = B - A X for one 8-column RHS chunk and 32 rows per workgroup (8 per
+// wave). X chunks of 128 rows are staged in LDS and shared by the 4 waves;
+// each lane owns two adjacent A columns per chunk (one 16-byte load per row).
+// The 64 partial sums per lane (8 rows x 8 columns) are combined with a
+// halving butterfly, after which lane L owns (row L/8, column L%8).
+constexpr int RES_ROWS = 32, RES_JC = 128, RES_NC = 8;
+__global__ __launch_bounds__(256) void residual_kernel(int n, int nrhs, const double* Ab, const double* Xb,
+                                                       const double* Bb, double* Rb, double* rmax, double* xmax,
+                                                       const int32_t* active) {
+  const int64_t mat = blockIdx.y;
+  if (active && active[mat] != 1) return;
+  const int c0 = blockIdx.z * RES_NC;
+  const int nc = min(RES_NC, nrhs - c0);
+  const double* __restrict__ A = Ab + mat * (int64_t)n * n;
+  const double* __restrict__ X = Xb + mat * (int64_t)n * nrhs;
+  const double* __restrict__ B = Bb + mat * (int64_t)n * nrhs;
+  double* __restrict__ R = Rb + mat * (int64_t)n * nrhs;
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int row0 = blockIdx.x * RES_ROWS + wave * 8;
+  __shared__ double xs[RES_JC][RES_NC];
+  double acc[8][RES_NC];
+#pragma unroll
+  for (int r = 0; r < 8; ++r)
+#pragma unroll
+    for (int c = 0; c < RES_NC; ++c) acc[r][c] = 0.0;
+  double xm = 0.0;
+  const bool even = (n & 1) == 0;
+  for (int jc = 0; jc < n; jc += RES_JC) {
+#pragma unroll
+    for (int q = 0; q < (RES_JC * RES_NC) / 256; ++q) {
+      const int e = t + 256 * q;
+      const int j = e / RES_NC, c = e % RES_NC;
+      const double v = (jc + j < n && c < nc) ? X[(int64_t)(jc + j) * nrhs + c0 + c] : 0.0;
+      xs[j][c] = v;
+      xm = fmax(xm, fabs(v));
+    }
+    __syncthreads();
+    const int j0 = jc + 2 * lane;
+    double x0[RES_NC], x1[RES_NC];
+#pragma unroll
+    for (int c = 0; c < RES_NC; ++c) { x0[c] = xs[2 * lane][c]; x1[c] = xs[2 * lane + 1][c]; }
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      const int row = row0 + r;
+      double a0 = 0.0, a1 = 0.0;
+      if (row < n) {
+        const double* ar = A + (int64_t)row * n;
+        if (even && j0 + 1 < n) {
+          const double2 v = *reinterpret_cast<const double2*>(ar + j0);
+          a0 = v.x; a1 = v.y;
+        } else {
+          a0 = j0 < n ? ar[j0] : 0.0;
+          a1 = j0 + 1 < n ? ar[j0 + 1] : 0.0;
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < RES_NC; ++c) acc[r][c] = fma(a1, x1[c], fma(a0, x0[c], acc[r][c]));
+    }
+    __syncthreads();
+  }
+  // Halving butterfly: 64 values over 64 lanes -> lane L holds value L.
+  double v[64];
+#pragma unroll
+  for (int r = 0; r < 8; ++r)
+#pragma unroll
+    for (int c = 0; c < RES_NC; ++c) v[r * RES_NC + c] = acc[r][c];
+#pragma unroll
+  for (int h = 32; h >= 1; h >>= 1) {
+    const bool up = (lane & h) != 0;
+#pragma unroll
+    for (int i = 0; i < h; ++i) {
+      const double send = up ? v[i] : v[i + h];
+      const double keep = up ? v[i + h] : v[i];
+      v[i] = keep + __shfl_xor(send, h, 64);
+    }
+  }
+  const int r = lane / RES_NC, c = lane % RES_NC;
+  const int row = row0 + r;
+  if (row < n && c < nc) {
+    const int64_t idx = (int64_t)row * nrhs + c0 + c;
+    const double res = B[idx] - v[0];
+    R[idx] = res;
+    atomic_max_nonneg(rmax + mat * nrhs + c0 + c, fabs(res));
+  }
+  if (blockIdx.x == 0) {
+    const int cc = t % RES_NC;
+    if (cc < nc) atomic_max_nonneg(xmax + mat * nrhs + c0 + cc, xm);
+  }
+}
+
+// Convergence test per system (one thread per system). Records eta per column,
+// decides active (1 = continue, 0 = converged, 2 = stopped without
+// convergence), computes the next sweep's scales, clears the max buffers and
+// counts systems still active.
+__global__ void eta_kernel(int64_t count, int nrhs, double tol, int max_iter, const double* anorm,
+                           const double* bmax, double* rmax, double* xmax, double* scale, double* prev,
+                           double* berr, int32_t* active, const int32_t* iters, int32_t* n_active) {
+  const int64_t mat = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (mat >= count || active[mat] != 1) return;
+  double worst = 0.0;
+  for (int c = 0; c < nrhs; ++c) {
+    const int64_t k = mat * nrhs + c;
+    const double rm = rmax[k], xm = xmax[k];
+    const double den = anorm[mat] * xm + bmax[k];
+    const double eta = den > 0.0 ? rm / den : (rm > 0.0 ? INFINITY : 0.0);
+    if (berr) berr[k] = eta;
+    worst = (eta > worst || isnan(eta)) ? eta : worst;
+    scale[k] = pow2_scale(rm);
+    rmax[k] = 0.0;
+    xmax[k] = 0.0;
+  }
+  if (worst <= tol) {
+    active[mat] = 0;
+  } else if (iters[mat] >= max_iter || !(worst < 0.5 * prev[mat])) {
+    active[mat] = 2;
+  } else {
+    atomicAdd(n_active, 1);
+  }
+  prev[mat] = worst;
+}
+
+__global__ void scale_from_bmax_kernel(int64_t n, const double* bmax, double* scale) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) scale[i] = pow2_scale(bmax[i]);
+}
+
+// Systems that did not converge (active == 2): mask for the FP64 fallback.
+__global__ void fallback_mask_kernel(int64_t count, const int32_t* active, int32_t* mask, int32_t* iters,
+                                     int32_t* n_fb) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  const int fb = active[i] == 2 ? 1 : 0;
+  mask[i] = fb;
+  if (fb) { iters[i] = -1; atomicAdd(n_fb, 1); }
+}
+
+__global__ void copy_masked_kernel(int64_t count, int64_t per, const double* src, double* dst, const int32_t* mask) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count * per) return;
+  if (mask[i / per]) dst[i] = src[i];
+}
+
+inline dim3 g1(int64_t n, int b = 256) { return dim3((unsigned)ceil_div(n, b)); }
+
+}  // namespace
+
+// Residual + per-(system, column) maxima of |r| and |x| for the current X.
+int residual_launch(int64_t count, int n, int nrhs, const double* A, const double* X, const double* B, double* R,
+                    double* rmax, double* xmax, const int32_t* active, hipStream_t s) {
+  dim3 grid((unsigned)ceil_div(n, RES_ROWS), (unsigned)count, (unsigned)ceil_div(nrhs, RES_NC));
+  ProfScope ps(kProfResidual, 2.0 * count * n * n * nrhs, 8.0 * count * ((double)n * n + 3.0 * n * nrhs), s);
+  hipLaunchKernelGGL(residual_kernel, grid, dim3(256), 0, s, n, nrhs, A, X, B, R, rmax, xmax, active);
+  NPPH_HIP(hipGetLastError());
+  return NPPH_OK;
+}
+
+int refined_impl(const npph_refine_opts& o, int64_t count, int n, int nrhs, const double* A, const double* B,
+                 double* X, double* berr, int32_t* iters_out, int32_t* info_out, hipStream_t s) {
+  if (count <= 0 || n <= 0 || nrhs <= 0) return NPPH_OK;
+  if (count > 65535) { set_error("refined solve: count > 65535"); return NPPH_ERR_INVALID; }
+  const int64_t per_a = (int64_t)n * n, per_b = (int64_t)n * nrhs;
+  const int64_t nb_all = count * per_b, nk = count * nrhs;
+
+  // Small bookkeeping arrays share one workspace slot.
+  const size_t dbl_cnt = (size_t)(5 * nk + 2 * count);  // anorm, bmax, rmax, xmax, scale, berr, prev
+  const size_t int_cnt = (size_t)(4 * count + 2);       // info, active, iters, mask, counters
+  char* misc;
+  NPPH_TRY(workspace(kWsNorm, dbl_cnt * 8 + int_cnt * 4 + 64, (void**)&misc));
+  double* anorm = (double*)misc;
+  double* bmax = anorm + count;
+  double* rmax = bmax + nk;
+  double* xmax = rmax + nk;
+  double* scale = xmax + nk;
+  double* berr_ws = scale + nk;
+  double* prev = berr_ws + nk;
+  int32_t* info = (int32_t*)(prev + count);
+  int32_t* active = info + count;
+  int32_t* iters = active + count;
+  int32_t* mask = iters + count;
+  int32_t* ctr = mask + count;  // [0] = active count, [1] = fallback count
+  double* berr_p = berr ? berr : berr_ws;
+  double* R;
+  NPPH_TRY(workspace(kWsR, sizeof(double) * (size_t)nb_all, (void**)&R));
+
+  hipLaunchKernelGGL(fill_kernel, g1(count + 3 * nk), dim3(256), 0, s, anorm, (int64_t)(count + 3 * nk), 0.0);
+  hipLaunchKernelGGL(fill_kernel, g1(count), dim3(256), 0, s, prev, count, (double)INFINITY);
+  hipLaunchKernelGGL(fill_kernel, g1(nk), dim3(256), 0, s, berr_p, nk, (double)NAN);
+  hipLaunchKernelGGL(zero_int_kernel, dim3(1), dim3(64), 0, s, ctr, 2);
+  hipLaunchKernelGGL(bnorm_kernel, g1(nb_all), dim3(256), 0, s, count, n, nrhs, B, bmax);
+  NPPH_HIP(hipGetLastError());
+
+  auto fp64_solve = [&](bool masked) -> int {
+    double *Aw, *Bw;
+    int32_t* ipiv;
+    NPPH_TRY(workspace(kWsA, sizeof(double) * (size_t)(count * per_a), (void**)&Aw));
+    NPPH_TRY(workspace(kWsB, sizeof(double) * (size_t)nb_all, (void**)&Bw));
+    NPPH_TRY(workspace(kWsPiv, sizeof(int32_t) * (size_t)(count * n), (void**)&ipiv));
+    int32_t* info64;
+    NPPH_TRY(workspace(kWsInfo, sizeof(int32_t) * (size_t)count, (void**)&info64));
+    NPPH_HIP(hipMemcpyAsync(Aw, A, sizeof(double) * count * per_a, hipMemcpyDeviceToDevice, s));
+    NPPH_HIP(hipMemcpyAsync(Bw, B, sizeof(double) * nb_all, hipMemcpyDeviceToDevice, s));
+    NPPH_TRY(gesv_dev_impl(count, n, nrhs, Aw, per_a, Bw, per_b, ipiv, info64, s));
+    if (masked) {
+      hipLaunchKernelGGL(copy_masked_kernel, g1(nb_all), dim3(256), 0, s, count, per_b, Bw, X, mask);
+      hipLaunchKernelGGL(copy_masked_int_kernel, g1(count), dim3(256), 0, s, count, info64, info, mask);
+    } else {
+      NPPH_HIP(hipMemcpyAsync(X, Bw, sizeof(double) * nb_all, hipMemcpyDeviceToDevice, s));
+      NPPH_HIP(hipMemcpyAsync(info, info64, sizeof(int32_t) * count, hipMemcpyDeviceToDevice, s));
+    }
+    NPPH_HIP(hipGetLastError());
+    return NPPH_OK;
+  };
+  // Reports eta for systems flagged in `active` (1) without further sweeps.
+  auto report = [&]() -> int {
+    NPPH_TRY(residual_launch(count, n, nrhs, A, X, B, R, rmax, xmax, active, s));
+    hipLaunchKernelGGL(eta_kernel, g1(count), dim3(256), 0, s, count, nrhs, INFINITY, 0, anorm, bmax, rmax, xmax,
+                       scale, prev, berr_p, active, iters, ctr);
+    NPPH_HIP(hipGetLastError());
+    return NPPH_OK;
+  };
+
+  if (o.precision == NPPH_PREC_FP64) {
+    hipLaunchKernelGGL(rowabs_kernel, g1(count * n * 64), dim3(256), 0, s, count, n, A, anorm);
+    NPPH_TRY(fp64_solve(false));
+    hipLaunchKernelGGL(init_state_kernel, g1(count), dim3(256), 0, s, count, info, active, iters);
+    NPPH_TRY(report());
+  } else {
+    if (o.precision != NPPH_PREC_FP32 && o.precision != NPPH_PREC_BF16) {
+      set_error("refined solve: unknown precision %d", o.precision);
+      return NPPH_ERR_INVALID;
+    }
+    const int terms = o.precision == NPPH_PREC_BF16 ? (o.bf16_terms > 0 ? o.bf16_terms : 6) : 0;
+    float *Alo, *Rlo, *Dlo;
+    int32_t *ipiv, *perm;
+    NPPH_TRY(workspace(kWsLo, sizeof(float) * (size_t)(count * per_a), (void**)&Alo));
+    NPPH_TRY(workspace(kWsRlo, sizeof(float) * (size_t)nb_all, (void**)&Rlo));
+    NPPH_TRY(workspace(kWsD, sizeof(float) * (size_t)nb_all, (void**)&Dlo));
+    NPPH_TRY(workspace(kWsPiv, sizeof(int32_t) * (size_t)(count * n), (void**)&ipiv));
+    NPPH_TRY(workspace(kWsPerm, sizeof(int32_t) * (size_t)(count * n), (void**)&perm));
+    hipLaunchKernelGGL(demote_kernel, g1(count * n * 64), dim3(256), 0, s, count, n, A, Alo, anorm);
+    NPPH_HIP(hipGetLastError());
+    NPPH_TRY(blocked_getrf_perm<float>(count, n, Alo, per_a, ipiv, perm, info, terms, nullptr, s));
+    hipLaunchKernelGGL(init_state_kernel, g1(count), dim3(256), 0, s, count, info, active, iters);
+    hipLaunchKernelGGL(scale_from_bmax_kernel, g1(nk), dim3(256), 0, s, nk, bmax, scale);
+    // Sweep 0 (x = 0, r = b).
+    hipLaunchKernelGGL(scale_demote_kernel, g1(nb_all), dim3(256), 0, s, count, n, nrhs, B, scale, Rlo, active);
+    NPPH_TRY(blocked_getrs_perm<float>(count, n, nrhs, Alo, per_a, perm, Rlo, per_b, Dlo, per_b, nullptr, active, s));
+    hipLaunchKernelGGL(update_kernel, g1(nb_all), dim3(256), 0, s, count, n, nrhs, X, Dlo, scale, active, 1);
+    hipLaunchKernelGGL(count_sweep_kernel, g1(count), dim3(256), 0, s, count, active, iters);
+    NPPH_HIP(hipGetLastError());
+    const int max_iter = o.max_iter > 0 ? o.max_iter : 10;
+    for (int sweep = 1;; ++sweep) {
+      NPPH_TRY(residual_launch(count, n, nrhs, A, X, B, R, rmax, xmax, active, s));
+      hipLaunchKernelGGL(zero_int_kernel, dim3(1), dim3(64), 0, s, ctr, 1);
+      hipLaunchKernelGGL(eta_kernel, g1(count), dim3(256), 0, s, count, nrhs, o.tol, max_iter, anorm, bmax, rmax,
+                         xmax, scale, prev, berr_p, active, iters, ctr);
+      NPPH_HIP(hipGetLastError());
+      int32_t n_act = 0;
+      NPPH_HIP(hipMemcpyAsync(&n_act, ctr, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+      NPPH_HIP(hipStreamSynchronize(s));
+      if (n_act == 0) break;
+      hipLaunchKernelGGL(scale_demote_kernel, g1(nb_all), dim3(256), 0, s, count, n, nrhs, R, scale, Rlo, active);
+      NPPH_TRY(blocked_getrs_perm<float>(count, n, nrhs, Alo, per_a, perm, Rlo, per_b, Dlo, per_b, nullptr, active, s));
+      hipLaunchKernelGGL(update_kernel, g1(nb_all), dim3(256), 0, s, count, n, nrhs, X, Dlo, scale, active, 0);
+      hipLaunchKernelGGL(count_sweep_kernel, g1(count), dim3(256), 0, s, count, active, iters);
+      NPPH_HIP(hipGetLastError());
+    }
+    if (o.fallback_fp64) {
+      hipLaunchKernelGGL(fallback_mask_kernel, g1(count), dim3(256), 0, s, count, active, mask, iters, ctr + 1);
+      int32_t n_fb = 0;
+      NPPH_HIP(hipMemcpyAsync(&n_fb, ctr + 1, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+      NPPH_HIP(hipStreamSynchronize(s));
+      if (n_fb > 0) {
+        NPPH_TRY(fp64_solve(true));
+        // eta of the fallback results (active = mask).
+        NPPH_HIP(hipMemcpyAsync(active, mask, sizeof(int32_t) * count, hipMemcpyDeviceToDevice, s));
+        NPPH_TRY(report());
+      }
+    }
+  }
+  if (iters_out) NPPH_HIP(hipMemcpyAsync(iters_out, iters, sizeof(int32_t) * count, hipMemcpyDeviceToDevice, s));
+  if (info_out) NPPH_HIP(hipMemcpyAsync(info_out, info, sizeof(int32_t) * count, hipMemcpyDeviceToDevice, s));
+  return NPPH_OK;
+}
+
+}  // namespace npph
+
+extern "C" int npph_gesv_refined_batched_dev(const npph_refine_opts* opts, int64_t count, int64_t n, int64_t nrhs,
+                                             const double* A, const double* B, double* X, double* berr,
+                                             int32_t* iters, int32_t* info, void* stream) {
+  NPPH_TRY(npph::ensure_init());
+  if (!opts || count < 0 || n < 0 || nrhs < 0 || n > npph::max_blocked_n()) {
+    npph::set_error("refined solve: bad arguments");
+    return NPPH_ERR_INVALID;
+  }
+  return npph::refined_impl(*opts, count, (int)n, (int)nrhs, A, B, X, berr, iters, info,
+                            stream ? (hipStream_t)stream : npph::default_stream());
+}

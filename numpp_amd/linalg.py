@@ -1,0 +1,142 @@
+"""Host-side mirror of numpp::linalg::solve with the HIP stacked-solve hook.
+
+Semantics follow NumPP (src/linalg/linalg.cpp, src/linalg/batched.cpp):
+  - ndim > 2 routes to the stacked solve (linalg.cpp:369); the right-hand side
+    is a stack of vectors when b.ndim == a.ndim - 1 (batched.cpp:92);
+  - the matrix must be square (linalg.cpp:30-33) -> LinAlgError;
+  - computation is float64; the result is float32 only when the promoted input
+    dtype is float32 (la_kind, linalg.cpp:20-26);
+  - an exact-zero pivot raises LinAlgError("Singular matrix") (linalg.cpp:100).
+The GPU hook is gated like matmul's GPU branch (backend.cpp:144-152): it runs
+when the target (forced argument, else NUMPP_GPU_TARGET) is Backend.Hip, calls
+the provider's gesv_batched slot once for the whole stack, and records
+last_backend. Otherwise the caller's `cpu_kernel` (NumPP's per-matrix loop in
+an integration) runs, or CpuPathSelected is raised.
+
+Also exposes the device-resident API (getrf/getrs/gesv and the mixed-precision
+refined solve) for callers that keep matrices in HBM.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib, vtable
+from .backend import Backend, CpuPathSelected, _set_last, env_target, gpu_available, is_gpu, backend_name
+from .device import DeviceArray
+
+
+class LinAlgError(ValueError):
+    """Mirror of numpp::linalg_error (include/numpp/core/error.hpp:43-46)."""
+
+
+def _out_dtype(a, b):
+    rt = np.result_type(a.dtype, b.dtype)
+    return np.float32 if rt == np.float32 else np.float64
+
+
+def solve(a, b, forced: Backend = Backend.Auto, cpu_kernel=None):
+    a = np.asarray(a)
+    b = np.asarray(b)
+    if a.ndim < 2 or a.shape[-1] != a.shape[-2]:
+        raise LinAlgError("Last 2 dimensions of the array must be square")
+    if np.iscomplexobj(a) or np.iscomplexobj(b):
+        raise NotImplementedError("complex solve is served by NumPP's CPU path")
+    out_dt = _out_dtype(a, b)
+    lead = a.shape[:-2]
+    n = a.shape[-1]
+    count = int(np.prod(lead, dtype=np.int64)) if lead else 1
+    bvec = (b.ndim == a.ndim - 1) if a.ndim > 2 else (b.ndim == 1)
+    nrhs = 1 if bvec else b.shape[-1]
+    exp_b = lead + ((n,) if bvec else (n, nrhs))
+    if b.shape != exp_b:
+        raise ValueError(f"solve: b has shape {b.shape}, expected {exp_b}")
+    target = forced if forced != Backend.Auto else env_target()
+    if is_gpu(target) and not gpu_available(target):
+        raise NotImplementedError(f"backend '{backend_name(target)}' is not available in this build")
+    if is_gpu(target):
+        vt = vtable.get()
+        if vt is not None and vt.gesv_batched:
+            A = np.ascontiguousarray(a, dtype=np.float64).reshape(count, n, n).copy()
+            B = np.ascontiguousarray(b, dtype=np.float64).reshape(count, n, nrhs).copy()
+            info = np.zeros(count, np.int32)
+            if vt.gesv_batched(_lib.DT_F64, count, n, nrhs, A.ctypes.data, B.ctypes.data, None,
+                               info.ctypes.data):
+                _set_last(target)
+                if np.any(info != 0):
+                    raise LinAlgError("Singular matrix")
+                return B.reshape(exp_b).astype(out_dt, copy=False)
+    _set_last(Backend.Cpu)
+    if cpu_kernel is None:
+        raise CpuPathSelected("solve dispatched to NumPP's CPU kernel")
+    return cpu_kernel(a, b).astype(out_dt, copy=False)
+
+
+# ---------------------------------------------------------------------------
+# Device-resident API
+
+
+def _ptr(x):
+    return None if x is None else x.ptr
+
+
+def gesv_batched_dev(A: DeviceArray, B: DeviceArray, ipiv: DeviceArray | None = None,
+                     info: DeviceArray | None = None, stream=None):
+    """In place: A (count, n, n) -> LU, B (count, n, nrhs) -> X (FP64)."""
+    lib = _lib.require_device()
+    count, n, _ = A.shape
+    nrhs = B.shape[2] if len(B.shape) == 3 else 1
+    _lib.check(lib.npph_gesv_batched_dev(count, n, nrhs, A.ptr, B.ptr, _ptr(ipiv), _ptr(info), stream))
+
+
+def getrf_batched_dev(A: DeviceArray, ipiv: DeviceArray, info: DeviceArray, stream=None):
+    lib = _lib.require_device()
+    count, n, _ = A.shape
+    _lib.check(lib.npph_getrf_batched_dev(count, n, A.ptr, n * n, ipiv.ptr, info.ptr, stream))
+
+
+def getrs_batched_dev(LU: DeviceArray, ipiv: DeviceArray, B: DeviceArray, stream=None):
+    lib = _lib.require_device()
+    count, n, _ = LU.shape
+    nrhs = B.shape[2] if len(B.shape) == 3 else 1
+    _lib.check(lib.npph_getrs_batched_dev(count, n, nrhs, LU.ptr, n * n, ipiv.ptr, B.ptr, n * nrhs, stream))
+
+
+PRECISIONS = {"fp64": _lib.PREC_FP64, "fp32": _lib.PREC_FP32, "bf16": _lib.PREC_BF16}
+
+
+def refine_opts(precision="fp32", tol=1e-12, max_iter=10, fallback=True, bf16_terms=6):
+    return _lib.RefineOpts(PRECISIONS[precision] if isinstance(precision, str) else int(precision),
+                           int(max_iter), float(tol), 1 if fallback else 0, int(bf16_terms))
+
+
+def gesv_refined_dev(A: DeviceArray, B: DeviceArray, X: DeviceArray, berr: DeviceArray | None = None,
+                     iters: DeviceArray | None = None, info: DeviceArray | None = None, opts=None,
+                     stream=None):
+    """Mixed-precision factorisation + FP64 refinement, device-resident."""
+    lib = _lib.require_device()
+    count, n, _ = A.shape
+    nrhs = B.shape[2] if len(B.shape) == 3 else 1
+    o = opts or refine_opts()
+    _lib.check(lib.npph_gesv_refined_batched_dev(ctypes.byref(o), count, n, nrhs, A.ptr, B.ptr, X.ptr,
+                                                 _ptr(berr), _ptr(iters), _ptr(info), stream))
+
+
+def solve_refined(a, b, precision="fp32", tol=1e-12, max_iter=10, fallback=True, bf16_terms=6):
+    """Host convenience: returns (x, berr, iters, info) for a (count, n, n) stack."""
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    squeeze = b.ndim == a.ndim - 1
+    if squeeze:
+        b = b[..., None]
+    count, n, _ = a.shape
+    nrhs = b.shape[2]
+    dA, dB = DeviceArray.from_host(a), DeviceArray.from_host(b)
+    dX = DeviceArray((count, n, nrhs))
+    dE = DeviceArray((count, nrhs))
+    dI = DeviceArray((count,), np.int32)
+    dF = DeviceArray((count,), np.int32)
+    gesv_refined_dev(dA, dB, dX, dE, dI, dF, refine_opts(precision, tol, max_iter, fallback, bf16_terms))
+    x = dX.to_host()
+    return (x[..., 0] if squeeze else x), dE.to_host(), dI.to_host(), dF.to_host()

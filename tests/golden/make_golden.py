@@ -1,0 +1,97 @@
+"""Writes the golden fixtures under tests/golden/ with NumPy/SciPy.
+
+NumPP's oracle is live NumPy (tests/oracle.hpp:46-79 in the reference); these
+fixtures freeze what NumPy 2.2 / SciPy 1.15 (LAPACK dgesv / dgetrf) return on
+the same inputs, so the CPU oracle and the HIP path can be checked offline.
+
+Fixtures (all .npz / .npy, loadable with allow_pickle=False):
+  gesv_cases.npz      seeded systems of many sizes with pivoting forced:
+                      A, B, X = np.linalg.solve(A, B), LAPACK ipiv (0-based
+                      pivot row per step) and packed LU from
+                      scipy.linalg.lu_factor.
+  config1_x.npy       np.linalg.solve on the 256 x 64 x 64 stacked workload
+                      (A = U[-1,1) + 64 I, b ~ N(0,1), default_rng(0)).
+  matmul_cases.npz    seeded float64 / float32 GEMMs with numpy results.
+  numpp_g15653376641833044941.npy
+                      the reference's own frozen matmul golden
+                      (arange(1,21).reshape(4,5) @ arange(2,17).reshape(5,3)),
+                      copied as data from the reference's tests/golden/.
+
+Run: python tests/golden/make_golden.py   (needs numpy + scipy, no GPU)
+"""
+from __future__ import annotations
+
+import os
+import shutil
+
+import numpy as np
+import scipy.linalg as sla
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def gesv_cases():
+    rng = np.random.default_rng(2024)
+    sizes = [1, 2, 3, 4, 5, 7, 8, 13, 16, 17, 31, 32, 33, 48, 63, 64, 65, 100, 129]
+    out = {}
+    for idx, n in enumerate(sizes):
+        nrhs = [1, 3, 8][idx % 3]
+        # Random matrix whose rows are shuffled and scaled so that partial
+        # pivoting has to swap rows; entries stay O(1) so the pivot gaps are wide.
+        a = rng.uniform(-1.0, 1.0, (n, n))
+        a += np.diag(rng.uniform(2.0, 3.0, n) * np.sign(rng.uniform(-1, 1, n)))
+        perm = rng.permutation(n)
+        a = a[perm]
+        b = rng.standard_normal((n, nrhs))
+        x = np.linalg.solve(a, b)
+        lu, ipiv = sla.lu_factor(a)
+        out[f"A_{n}"] = a
+        out[f"B_{n}"] = b
+        out[f"X_{n}"] = x
+        out[f"LU_{n}"] = lu
+        out[f"ipiv_{n}"] = ipiv.astype(np.int32)
+    out["sizes"] = np.array(sizes, np.int32)
+    np.savez(os.path.join(HERE, "gesv_cases.npz"), **out)
+
+
+def config1():
+    rng = np.random.default_rng(0)
+    a = rng.uniform(-1.0, 1.0, (256, 64, 64)) + 64.0 * np.eye(64)
+    b = rng.standard_normal((256, 64))
+    x = np.linalg.solve(a, b[..., None])[..., 0]
+    np.save(os.path.join(HERE, "config1_x.npy"), x)
+
+
+def matmul_cases():
+    rng = np.random.default_rng(1)
+    out = {}
+    shapes = [(1, 1, 1), (2, 3, 2), (37, 53, 29), (64, 64, 64), (65, 17, 130), (96, 80, 72)]
+    for t, (m, k, n) in enumerate(shapes):
+        a = rng.standard_normal((m, k))
+        b = rng.standard_normal((k, n))
+        out[f"A64_{t}"] = a
+        out[f"B64_{t}"] = b
+        out[f"C64_{t}"] = a @ b
+        a32 = a.astype(np.float32)
+        b32 = b.astype(np.float32)
+        out[f"A32_{t}"] = a32
+        out[f"B32_{t}"] = b32
+        out[f"C32_{t}"] = a32 @ b32
+    out["count"] = np.array([len(shapes)], np.int32)
+    np.savez(os.path.join(HERE, "matmul_cases.npz"), **out)
+
+
+def reference_golden():
+    src = "/root/reference/tests/golden/g15653376641833044941.npy"
+    dst = os.path.join(HERE, "numpp_g15653376641833044941.npy")
+    if os.path.exists(src):
+        arr = np.load(src, allow_pickle=False)
+        np.save(dst, arr)
+
+
+if __name__ == "__main__":
+    gesv_cases()
+    config1()
+    matmul_cases()
+    reference_golden()
+    print("goldens written to", HERE)

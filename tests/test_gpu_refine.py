@@ -1,0 +1,92 @@
+"""Mixed-precision factorisation + FP64 refinement on the GPU.
+
+Checks (tolerances stated): every result's normwise backward error, recomputed
+on the CPU by the oracle with a Dot2 residual, is <= 1e-12; the forward error
+against the oracle's FP64 solution is <= 1e-12 * kappa; the device-reported eta
+agrees with the oracle's; systems that cannot converge fall back to FP64
+(iters = -1) and still meet the bound; and a system gives bitwise the same
+result alone and inside a larger stack.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from conftest import cond_matrix
+from numpp_amd import linalg as nla
+
+pytestmark = pytest.mark.gpu
+
+
+def make_stack(seed, count, n, nrhs, cond):
+    rng = np.random.default_rng(seed)
+    a = np.stack([cond_matrix(rng, n, cond) for _ in range(count)])
+    b = rng.standard_normal((count, n, nrhs))
+    return a, b
+
+
+def check_results(a, b, x, berr, tol=1e-12, cond=1e4):
+    for i in range(a.shape[0]):
+        eta = oracle.backward_error(a[i], b[i], x[i])
+        assert np.all(eta <= tol), (i, eta)
+        # device eta and oracle eta agree to a factor (both tiny, different summation)
+        assert np.all(berr[i] <= tol)
+        xo = oracle.solve(a[i], b[i])
+        fe = np.abs(x[i] - xo).max() / np.abs(xo).max()
+        assert fe <= 1e-12 * cond, (i, fe)
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16"])
+@pytest.mark.parametrize("n,nrhs", [(96, 3), (256, 8), (300, 1)])
+def test_refined_converges(hip, precision, n, nrhs):
+    a, b = make_stack(100 + n, 3, n, nrhs, 1e4)
+    x, berr, iters, info = nla.solve_refined(a, b, precision=precision)
+    assert np.all(info == 0)
+    assert np.all(iters >= 1), iters
+    check_results(a, b, x, berr)
+
+
+def test_fp64_precision_reports(hip):
+    a, b = make_stack(7, 2, 128, 2, 1e4)
+    x, berr, iters, info = nla.solve_refined(a, b, precision="fp64")
+    assert np.all(iters == 0)
+    check_results(a, b, x, berr)
+
+
+def test_plain_bf16_falls_back(hip):
+    # One-term BF16 (8-bit significand) at kappa = 1e6 cannot refine to 1e-12:
+    # the system must stop (stagnation / max_iter) and be re-solved in FP64.
+    a, b = make_stack(8, 2, 200, 2, 1e6)
+    x, berr, iters, info = nla.solve_refined(a, b, precision="bf16", bf16_terms=1, max_iter=4)
+    assert np.all(iters == -1), iters
+    check_results(a, b, x, berr, cond=1e6)
+
+
+def test_no_fallback_reports_failure(hip):
+    a, b = make_stack(9, 1, 200, 1, 1e6)
+    x, berr, iters, info = nla.solve_refined(a, b, precision="bf16", bf16_terms=1, max_iter=2, fallback=False)
+    assert iters[0] >= 1 and berr[0, 0] > 1e-12
+
+
+def test_alone_equals_in_mix(hip):
+    a, b = make_stack(10, 6, 160, 2, 1e4)
+    for precision in ("fp32", "bf16", "fp64"):
+        xs, _, _, _ = nla.solve_refined(a, b, precision=precision)
+        for i in (0, 3, 5):
+            x1, _, _, _ = nla.solve_refined(a[i:i + 1], b[i:i + 1], precision=precision)
+            np.testing.assert_array_equal(x1[0], xs[i])
+
+
+def test_vtable_refined_slot(hip):
+    from numpp_amd import vtable
+    a, b = make_stack(12, 2, 100, 2, 1e3)
+    vt = vtable.get()
+    x = np.empty_like(b)
+    berr = np.empty((2, 2))
+    iters = np.empty(2, np.int32)
+    info = np.empty(2, np.int32)
+    a_c, b_c = a.copy(), b.copy()
+    ok = vt.gesv_refined_batched(1, 11, 2, 100, 2, a_c.ctypes.data, b_c.ctypes.data, x.ctypes.data,
+                                 berr.ctypes.data, iters.ctypes.data, info.ctypes.data)
+    assert ok
+    np.testing.assert_array_equal(a_c, a)  # inputs untouched
+    check_results(a, b, x, berr, cond=1e3)
