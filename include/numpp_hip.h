@@ -147,6 +147,68 @@ int npph_gesv_refined_batched_dev(const npph_refine_opts* opts, int64_t count, i
                                   int64_t nrhs, const double* A, const double* B, double* X,
                                   double* berr, int32_t* iters, int32_t* info, void* stream);
 
+/* ---------------- uneven batches ---------------- */
+/* Many independent systems of different order n_i and nrhs_i in one call. No
+ * reference counterpart (a NumPP stack must be uniform, batched.cpp:22-30); the
+ * per-system semantics are those of npph_gesv_batched / the refined solver.
+ * Packed layout: system i's A starts at element sum_{j<i} n_j^2, its B and X at
+ * sum_{j<i} n_j*nrhs_j, its berr at sum_{j<i} nrhs_j; all row-major.
+ * plan_create sorts the systems into padded size buckets (identity padding;
+ * granularity chosen by a measured autotuner when `autotune` is 1, cached in
+ * $NUMPP_HIP_TUNE_CACHE or ~/.cache/numpp_hip_tune.json) and allocates the
+ * bucket buffers. execute packs, solves every bucket and unpacks; with
+ * use_graph = 1 and FP64 precision the whole sequence is captured once into a
+ * hipGraph (first call runs eagerly to size workspaces) and replayed.
+ * n_i in [1, 4096], nrhs_i in [1, 4096]; host arrays n, nrhs. */
+int npph_vbatch_plan_create(int64_t count, const int32_t* n, const int32_t* nrhs,
+                            const npph_refine_opts* opts, int autotune, void** plan);
+int npph_vbatch_plan_destroy(void* plan);
+int npph_vbatch_plan_info(void* plan, int* nbuckets, int32_t* order, int64_t* counts, int32_t* rpad,
+                          int cap);
+int npph_vbatch_execute(void* plan, const double* A, const double* B, double* X, double* berr,
+                        int32_t* iters, int32_t* info, int use_graph, void* stream);
+
+/* ---------------- serving loop ---------------- */
+/* Request loop for solves of different n and nrhs arriving over time. No
+ * reference counterpart (NumPP's calls are synchronous and keep nothing on the
+ * device, docs/gpu-backends.md:30-31). W worker threads, each with its own
+ * stream and workspace, take requests from a FIFO queue when their device
+ * bytes fit in the budget (resident factors are evicted LRU to make room);
+ * queued one-shot requests of the same order are solved together as one stack
+ * (continuous batching; results are bitwise the same alone and in the mix).
+ * keep = 1 with a nonzero key leaves the factorisation resident; a later
+ * request with that key and A = NULL solves with the resident factors.
+ * Host buffers A, B, X, berr must stay valid until npph_server_wait returns. */
+typedef struct npph_serve_config {
+  double budget_bytes;  /* device byte budget for admitted work + resident factors */
+  int workers;          /* concurrent worker streams (1..16) */
+  int max_batch;        /* max same-order requests solved as one stack */
+  int precision;        /* NPPH_PREC_* */
+  int max_iter;
+  double tol;
+  int fallback_fp64;
+  int bf16_terms;
+} npph_serve_config;
+int npph_server_create(const npph_serve_config* cfg, void** srv);
+int npph_server_destroy(void* srv);
+int npph_server_submit(void* srv, int64_t n, int64_t nrhs, const double* A, const double* B, double* X,
+                       double* berr, uint64_t key, int keep, int64_t* req_id);
+/* Blocks until the request is done; returns its status (NPPH_ERR_BUDGET when
+ * it can never fit). latency_ms = submit to results on the host; queue_ms =
+ * submit to admission. */
+int npph_server_wait(void* srv, int64_t req_id, int32_t* iters, int32_t* info, double* latency_ms,
+                     double* queue_ms);
+int npph_server_stats(void* srv, int64_t* completed, int64_t* resident, double* bytes_in_use,
+                      double* peak_bytes, int64_t* batched, int64_t* fallbacks, int64_t* evictions);
+/* Replays a trace: request i is submitted t_arrive[i] seconds after the call
+ * starts; per-request latency, iters, info and status are returned and
+ * wall_s is the first-submit-to-last-completion time. */
+int npph_server_replay(void* srv, int64_t count, const double* t_arrive, const int32_t* n,
+                       const int32_t* nrhs, const double* const* A, const double* const* B,
+                       double* const* X, double* const* berr, const uint64_t* key,
+                       const int32_t* keep, double* latency_ms, int32_t* iters, int32_t* info,
+                       int32_t* status, double* wall_s);
+
 /* ---------------- profiling ---------------- */
 /* Per-kernel-class timing with HIP events around every launch (off by default;
  * no reference counterpart: NumPP has no timers or benchmarks).

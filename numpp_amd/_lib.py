@@ -38,6 +38,19 @@ class HipError(RuntimeError):
         self.status = status
 
 
+class ServeConfig(ctypes.Structure):
+    _fields_ = [
+        ("budget_bytes", ctypes.c_double),
+        ("workers", ctypes.c_int),
+        ("max_batch", ctypes.c_int),
+        ("precision", ctypes.c_int),
+        ("max_iter", ctypes.c_int),
+        ("tol", ctypes.c_double),
+        ("fallback_fp64", ctypes.c_int),
+        ("bf16_terms", ctypes.c_int),
+    ]
+
+
 class RefineOpts(ctypes.Structure):
     _fields_ = [
         ("precision", ctypes.c_int),
@@ -84,6 +97,16 @@ SIGNATURES = {
     "npph_getrs_batched_dev": (_int, [_i64, _i64, _i64, _vp, _i64, _vp, _vp, _i64, _vp]),
     "npph_gesv_batched_dev": (_int, [_i64, _i64, _i64, _vp, _vp, _vp, _vp, _vp]),
     "npph_gesv_refined_batched_dev": (_int, [ctypes.POINTER(RefineOpts), _i64, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "npph_vbatch_plan_create": (_int, [_i64, _vp, _vp, ctypes.POINTER(RefineOpts), _int, ctypes.POINTER(_vp)]),
+    "npph_vbatch_plan_destroy": (_int, [_vp]),
+    "npph_vbatch_plan_info": (_int, [_vp, ctypes.POINTER(_int), _vp, _vp, _vp, _int]),
+    "npph_vbatch_execute": (_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _int, _vp]),
+    "npph_server_create": (_int, [ctypes.POINTER(ServeConfig), ctypes.POINTER(_vp)]),
+    "npph_server_destroy": (_int, [_vp]),
+    "npph_server_submit": (_int, [_vp, _i64, _i64, _vp, _vp, _vp, _vp, ctypes.c_uint64, _int, ctypes.POINTER(_i64)]),
+    "npph_server_wait": (_int, [_vp, _i64, _vp, _vp, ctypes.POINTER(_dbl), ctypes.POINTER(_dbl)]),
+    "npph_server_stats": (_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "npph_server_replay": (_int, [_vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.POINTER(_dbl)]),
     "npph_prof_enable": (_int, [_int]),
     "npph_prof_reset": (_int, []),
     "npph_prof_get": (_int, [_int, ctypes.POINTER(_dbl), ctypes.POINTER(_dbl), ctypes.POINTER(_dbl), ctypes.POINTER(_i64)]),

@@ -48,10 +48,22 @@ int cu_count();                   // compute units on the device
 // (grow-only, freed at process exit). Slots let concurrent phases of one call
 // hold separate buffers; callers that share a slot must not overlap.
 int workspace(int slot, size_t bytes, void** out);
+// Incremented whenever a workspace slot is reallocated; a captured hipGraph
+// that baked in workspace addresses is stale once this changes.
+uint64_t workspace_generation();
+// True while `s` is being captured into a hipGraph (no allocation, no sync).
+bool stream_capturing(hipStream_t s);
 enum WsSlot { kWsA = 0, kWsB, kWsPiv, kWsInfo, kWsLo, kWsR, kWsX, kWsNorm, kWsMisc, kWsMove, kWsPerm, kWsD, kWsRlo, kWsCount };
 
-// Grow-only helper arena owned by the caller (used by the serving loop and the
-// uneven-batch plan so their buffers do not alias the per-call slots).
+// A private set of workspace slots. A thread that binds one (ws_bind) gets its
+// own buffers from workspace(), so calls running concurrently on different
+// streams from different threads never share scratch memory; growth of a
+// bound set waits only for the set's owner stream.
+struct WsSet;
+WsSet* ws_create(hipStream_t owner);
+void ws_destroy(WsSet* w);
+void ws_bind(WsSet* w);  // nullptr = the process-wide set
+
 inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
 // ---- wave-level helpers ----

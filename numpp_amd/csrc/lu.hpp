@@ -63,4 +63,31 @@ int blocked_getrs_perm(int64_t count, int n, int nrhs, const T* LU, int64_t stri
 // perm from LAPACK-style pivots (replays the swaps).
 int ipiv_to_perm(int64_t count, int n, const int32_t* ipiv, int32_t* perm, hipStream_t s);
 
+// Factors of a uniform stack for the refined solver (device memory carved from
+// one buffer by refine_factor_carve). lu64 is used for FP64 precision, lu32
+// for the FP32 / split-BF16 precisions.
+struct RefineFactors {
+  int precision = NPPH_PREC_FP64;
+  int64_t count = 0;
+  int n = 0;
+  double* lu64 = nullptr;
+  float* lu32 = nullptr;
+  int32_t* ipiv = nullptr;
+  int32_t* perm = nullptr;
+  int32_t* info = nullptr;
+  double* anorm = nullptr;
+};
+size_t refine_factor_bytes(int precision, int64_t count, int n);
+void refine_factor_carve(int precision, int64_t count, int n, void* buf, RefineFactors& f);
+// Factorises A (count x n x n FP64, unchanged) into f in f.precision.
+int refine_factor(const npph_refine_opts& o, const double* A, RefineFactors& f, hipStream_t s);
+// Solves with resident factors, refining in FP64 against A (needed for the
+// residual) until every column meets o.tol; B and X must not alias.
+int refine_solve(const npph_refine_opts& o, const double* A, const RefineFactors& f, int nrhs, const double* B,
+                 double* X, double* berr, int32_t* iters_out, int32_t* info_out, hipStream_t s);
+int refined_impl(const npph_refine_opts& o, int64_t count, int n, int nrhs, const double* A, const double* B,
+                 double* X, double* berr, int32_t* iters_out, int32_t* info_out, hipStream_t s);
+int gesv_dev_impl(int64_t count, int n, int nrhs, double* A, int64_t strideA, double* B, int64_t strideB,
+                  int32_t* ipiv, int32_t* info, hipStream_t s);
+
 }  // namespace npph

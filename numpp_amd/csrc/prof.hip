@@ -32,6 +32,7 @@ hipEvent_t take_event() {
 bool prof_enabled() { return g_on.load(std::memory_order_relaxed) != 0; }
 
 void prof_begin(int kind, double flops, double bytes, hipStream_t s, void** token) {
+  if (stream_capturing(s)) return;  // no event records inside a captured graph
   std::lock_guard<std::mutex> g(g_mu);
   Rec r{kind, flops, bytes, take_event(), take_event()};
   if (!r.a || !r.b) return;

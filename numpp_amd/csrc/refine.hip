@@ -282,20 +282,80 @@ int residual_launch(int64_t count, int n, int nrhs, const double* A, const doubl
   return NPPH_OK;
 }
 
-int refined_impl(const npph_refine_opts& o, int64_t count, int n, int nrhs, const double* A, const double* B,
-                 double* X, double* berr, int32_t* iters_out, int32_t* info_out, hipStream_t s) {
-  if (count <= 0 || n <= 0 || nrhs <= 0) return NPPH_OK;
+// Bytes of factor storage for a uniform stack (see RefineFactors).
+size_t refine_factor_bytes(int precision, int64_t count, int n) {
+  const size_t nn = (size_t)count * n * n;
+  const size_t lu = precision == NPPH_PREC_FP64 ? nn * 8 : nn * 4;
+  return lu + 2 * sizeof(int32_t) * (size_t)count * n + sizeof(int32_t) * count + 8 * (size_t)count + 1024;
+}
+
+// Carves a RefineFactors out of one caller-owned device buffer of
+// refine_factor_bytes() bytes.
+void refine_factor_carve(int precision, int64_t count, int n, void* buf, RefineFactors& f) {
+  char* p = static_cast<char*>(buf);
+  auto take = [&](size_t bytes) {
+    char* q = p;
+    p += (bytes + 255) & ~(size_t)255;
+    return q;
+  };
+  f.precision = precision;
+  f.count = count;
+  f.n = n;
+  const size_t nn = (size_t)count * n * n;
+  f.lu64 = nullptr;
+  f.lu32 = nullptr;
+  if (precision == NPPH_PREC_FP64) f.lu64 = (double*)take(nn * 8);
+  else f.lu32 = (float*)take(nn * 4);
+  f.anorm = (double*)take(8 * (size_t)count);
+  f.ipiv = (int32_t*)take(sizeof(int32_t) * (size_t)count * n);
+  f.perm = (int32_t*)take(sizeof(int32_t) * (size_t)count * n);
+  f.info = (int32_t*)take(sizeof(int32_t) * (size_t)count);
+}
+
+int refine_factor(const npph_refine_opts& o, const double* A, RefineFactors& f, hipStream_t s) {
+  const int64_t count = f.count;
+  const int n = f.n;
+  if (count <= 0 || n <= 0) return NPPH_OK;
   if (count > 65535) { set_error("refined solve: count > 65535"); return NPPH_ERR_INVALID; }
+  hipLaunchKernelGGL(fill_kernel, g1(count), dim3(256), 0, s, f.anorm, count, 0.0);
+  if (o.precision == NPPH_PREC_FP64) {
+    const int64_t per_a = (int64_t)n * n;
+    hipLaunchKernelGGL(rowabs_kernel, g1(count * n * 64), dim3(256), 0, s, count, n, A, f.anorm);
+    NPPH_HIP(hipMemcpyAsync(f.lu64, A, sizeof(double) * count * per_a, hipMemcpyDeviceToDevice, s));
+    if (n <= 64) {
+      UniformBatch u{count, n, f.lu64, per_a, f.lu64, f.ipiv, f.info, 0, nullptr, 0, nullptr};
+      NPPH_TRY(small_lu_launch(u, s));
+      NPPH_TRY(ipiv_to_perm(count, n, f.ipiv, f.perm, s));
+    } else {
+      NPPH_TRY(blocked_getrf_perm<double>(count, n, f.lu64, per_a, f.ipiv, f.perm, f.info, 0, nullptr, s));
+    }
+    NPPH_HIP(hipGetLastError());
+    return NPPH_OK;
+  }
+  if (o.precision != NPPH_PREC_FP32 && o.precision != NPPH_PREC_BF16) {
+    set_error("refined solve: unknown precision %d", o.precision);
+    return NPPH_ERR_INVALID;
+  }
+  const int terms = o.precision == NPPH_PREC_BF16 ? (o.bf16_terms > 0 ? o.bf16_terms : 6) : 0;
+  hipLaunchKernelGGL(demote_kernel, g1(count * n * 64), dim3(256), 0, s, count, n, A, f.lu32, f.anorm);
+  NPPH_HIP(hipGetLastError());
+  return blocked_getrf_perm<float>(count, n, f.lu32, (int64_t)n * n, f.ipiv, f.perm, f.info, terms, nullptr, s);
+}
+
+int refine_solve(const npph_refine_opts& o, const double* A, const RefineFactors& f, int nrhs, const double* B,
+                 double* X, double* berr, int32_t* iters_out, int32_t* info_out, hipStream_t s) {
+  const int64_t count = f.count;
+  const int n = f.n;
+  if (count <= 0 || n <= 0 || nrhs <= 0) return NPPH_OK;
   const int64_t per_a = (int64_t)n * n, per_b = (int64_t)n * nrhs;
   const int64_t nb_all = count * per_b, nk = count * nrhs;
 
-  // Small bookkeeping arrays share one workspace slot.
-  const size_t dbl_cnt = (size_t)(5 * nk + 2 * count);  // anorm, bmax, rmax, xmax, scale, berr, prev
-  const size_t int_cnt = (size_t)(4 * count + 2);       // info, active, iters, mask, counters
+  // Bookkeeping arrays share one workspace slot.
+  const size_t dbl_cnt = (size_t)(5 * nk + count);  // bmax, rmax, xmax, scale, berr, prev
+  const size_t int_cnt = (size_t)(4 * count + 2);   // info, active, iters, mask, counters
   char* misc;
   NPPH_TRY(workspace(kWsNorm, dbl_cnt * 8 + int_cnt * 4 + 64, (void**)&misc));
-  double* anorm = (double*)misc;
-  double* bmax = anorm + count;
+  double* bmax = (double*)misc;
   double* rmax = bmax + nk;
   double* xmax = rmax + nk;
   double* scale = xmax + nk;
@@ -309,21 +369,24 @@ int refined_impl(const npph_refine_opts& o, int64_t count, int n, int nrhs, cons
   double* berr_p = berr ? berr : berr_ws;
   double* R;
   NPPH_TRY(workspace(kWsR, sizeof(double) * (size_t)nb_all, (void**)&R));
+  const double* anorm = f.anorm;
 
-  hipLaunchKernelGGL(fill_kernel, g1(count + 3 * nk), dim3(256), 0, s, anorm, (int64_t)(count + 3 * nk), 0.0);
+  hipLaunchKernelGGL(fill_kernel, g1(3 * nk), dim3(256), 0, s, bmax, (int64_t)(3 * nk), 0.0);
   hipLaunchKernelGGL(fill_kernel, g1(count), dim3(256), 0, s, prev, count, (double)INFINITY);
   hipLaunchKernelGGL(fill_kernel, g1(nk), dim3(256), 0, s, berr_p, nk, (double)NAN);
   hipLaunchKernelGGL(zero_int_kernel, dim3(1), dim3(64), 0, s, ctr, 2);
   hipLaunchKernelGGL(bnorm_kernel, g1(nb_all), dim3(256), 0, s, count, n, nrhs, B, bmax);
+  NPPH_HIP(hipMemcpyAsync(info, f.info, sizeof(int32_t) * count, hipMemcpyDeviceToDevice, s));
   NPPH_HIP(hipGetLastError());
 
+  // Plain FP64 solve of the systems selected by `mask` (all when masked is
+  // false), from a fresh FP64 factorisation of A.
   auto fp64_solve = [&](bool masked) -> int {
     double *Aw, *Bw;
-    int32_t* ipiv;
+    int32_t *ipiv, *info64;
     NPPH_TRY(workspace(kWsA, sizeof(double) * (size_t)(count * per_a), (void**)&Aw));
     NPPH_TRY(workspace(kWsB, sizeof(double) * (size_t)nb_all, (void**)&Bw));
     NPPH_TRY(workspace(kWsPiv, sizeof(int32_t) * (size_t)(count * n), (void**)&ipiv));
-    int32_t* info64;
     NPPH_TRY(workspace(kWsInfo, sizeof(int32_t) * (size_t)count, (void**)&info64));
     NPPH_HIP(hipMemcpyAsync(Aw, A, sizeof(double) * count * per_a, hipMemcpyDeviceToDevice, s));
     NPPH_HIP(hipMemcpyAsync(Bw, B, sizeof(double) * nb_all, hipMemcpyDeviceToDevice, s));
@@ -338,7 +401,7 @@ int refined_impl(const npph_refine_opts& o, int64_t count, int n, int nrhs, cons
     NPPH_HIP(hipGetLastError());
     return NPPH_OK;
   };
-  // Reports eta for systems flagged in `active` (1) without further sweeps.
+  // eta of the systems flagged active (1), without further sweeps.
   auto report = [&]() -> int {
     NPPH_TRY(residual_launch(count, n, nrhs, A, X, B, R, rmax, xmax, active, s));
     hipLaunchKernelGGL(eta_kernel, g1(count), dim3(256), 0, s, count, nrhs, INFINITY, 0, anorm, bmax, rmax, xmax,
@@ -348,36 +411,24 @@ int refined_impl(const npph_refine_opts& o, int64_t count, int n, int nrhs, cons
   };
 
   if (o.precision == NPPH_PREC_FP64) {
-    hipLaunchKernelGGL(rowabs_kernel, g1(count * n * 64), dim3(256), 0, s, count, n, A, anorm);
-    NPPH_TRY(fp64_solve(false));
+    NPPH_TRY(blocked_getrs_perm<double>(count, n, nrhs, f.lu64, per_a, f.perm, B, per_b, X, per_b, f.info, nullptr, s));
     hipLaunchKernelGGL(init_state_kernel, g1(count), dim3(256), 0, s, count, info, active, iters);
     NPPH_TRY(report());
   } else {
-    if (o.precision != NPPH_PREC_FP32 && o.precision != NPPH_PREC_BF16) {
-      set_error("refined solve: unknown precision %d", o.precision);
-      return NPPH_ERR_INVALID;
-    }
-    const int terms = o.precision == NPPH_PREC_BF16 ? (o.bf16_terms > 0 ? o.bf16_terms : 6) : 0;
-    float *Alo, *Rlo, *Dlo;
-    int32_t *ipiv, *perm;
-    NPPH_TRY(workspace(kWsLo, sizeof(float) * (size_t)(count * per_a), (void**)&Alo));
+    float *Rlo, *Dlo;
     NPPH_TRY(workspace(kWsRlo, sizeof(float) * (size_t)nb_all, (void**)&Rlo));
     NPPH_TRY(workspace(kWsD, sizeof(float) * (size_t)nb_all, (void**)&Dlo));
-    NPPH_TRY(workspace(kWsPiv, sizeof(int32_t) * (size_t)(count * n), (void**)&ipiv));
-    NPPH_TRY(workspace(kWsPerm, sizeof(int32_t) * (size_t)(count * n), (void**)&perm));
-    hipLaunchKernelGGL(demote_kernel, g1(count * n * 64), dim3(256), 0, s, count, n, A, Alo, anorm);
-    NPPH_HIP(hipGetLastError());
-    NPPH_TRY(blocked_getrf_perm<float>(count, n, Alo, per_a, ipiv, perm, info, terms, nullptr, s));
     hipLaunchKernelGGL(init_state_kernel, g1(count), dim3(256), 0, s, count, info, active, iters);
     hipLaunchKernelGGL(scale_from_bmax_kernel, g1(nk), dim3(256), 0, s, nk, bmax, scale);
     // Sweep 0 (x = 0, r = b).
     hipLaunchKernelGGL(scale_demote_kernel, g1(nb_all), dim3(256), 0, s, count, n, nrhs, B, scale, Rlo, active);
-    NPPH_TRY(blocked_getrs_perm<float>(count, n, nrhs, Alo, per_a, perm, Rlo, per_b, Dlo, per_b, nullptr, active, s));
+    NPPH_TRY(blocked_getrs_perm<float>(count, n, nrhs, f.lu32, per_a, f.perm, Rlo, per_b, Dlo, per_b, nullptr, active,
+                                       s));
     hipLaunchKernelGGL(update_kernel, g1(nb_all), dim3(256), 0, s, count, n, nrhs, X, Dlo, scale, active, 1);
     hipLaunchKernelGGL(count_sweep_kernel, g1(count), dim3(256), 0, s, count, active, iters);
     NPPH_HIP(hipGetLastError());
     const int max_iter = o.max_iter > 0 ? o.max_iter : 10;
-    for (int sweep = 1;; ++sweep) {
+    for (;;) {
       NPPH_TRY(residual_launch(count, n, nrhs, A, X, B, R, rmax, xmax, active, s));
       hipLaunchKernelGGL(zero_int_kernel, dim3(1), dim3(64), 0, s, ctr, 1);
       hipLaunchKernelGGL(eta_kernel, g1(count), dim3(256), 0, s, count, nrhs, o.tol, max_iter, anorm, bmax, rmax,
@@ -388,7 +439,8 @@ int refined_impl(const npph_refine_opts& o, int64_t count, int n, int nrhs, cons
       NPPH_HIP(hipStreamSynchronize(s));
       if (n_act == 0) break;
       hipLaunchKernelGGL(scale_demote_kernel, g1(nb_all), dim3(256), 0, s, count, n, nrhs, R, scale, Rlo, active);
-      NPPH_TRY(blocked_getrs_perm<float>(count, n, nrhs, Alo, per_a, perm, Rlo, per_b, Dlo, per_b, nullptr, active, s));
+      NPPH_TRY(blocked_getrs_perm<float>(count, n, nrhs, f.lu32, per_a, f.perm, Rlo, per_b, Dlo, per_b, nullptr,
+                                         active, s));
       hipLaunchKernelGGL(update_kernel, g1(nb_all), dim3(256), 0, s, count, n, nrhs, X, Dlo, scale, active, 0);
       hipLaunchKernelGGL(count_sweep_kernel, g1(count), dim3(256), 0, s, count, active, iters);
       NPPH_HIP(hipGetLastError());
@@ -400,7 +452,6 @@ int refined_impl(const npph_refine_opts& o, int64_t count, int n, int nrhs, cons
       NPPH_HIP(hipStreamSynchronize(s));
       if (n_fb > 0) {
         NPPH_TRY(fp64_solve(true));
-        // eta of the fallback results (active = mask).
         NPPH_HIP(hipMemcpyAsync(active, mask, sizeof(int32_t) * count, hipMemcpyDeviceToDevice, s));
         NPPH_TRY(report());
       }
@@ -411,6 +462,17 @@ int refined_impl(const npph_refine_opts& o, int64_t count, int n, int nrhs, cons
   return NPPH_OK;
 }
 
+int refined_impl(const npph_refine_opts& o, int64_t count, int n, int nrhs, const double* A, const double* B,
+                 double* X, double* berr, int32_t* iters_out, int32_t* info_out, hipStream_t s) {
+  if (count <= 0 || n <= 0 || nrhs <= 0) return NPPH_OK;
+  void* buf;
+  NPPH_TRY(workspace(kWsLo, refine_factor_bytes(o.precision, count, n), &buf));
+  RefineFactors f;
+  refine_factor_carve(o.precision, count, n, buf, f);
+  NPPH_TRY(refine_factor(o, A, f, s));
+  return refine_solve(o, A, f, nrhs, B, X, berr, iters_out, info_out, s);
+}
+
 }  // namespace npph
 
 extern "C" int npph_gesv_refined_batched_dev(const npph_refine_opts* opts, int64_t count, int64_t n, int64_t nrhs,
@@ -419,6 +481,10 @@ extern "C" int npph_gesv_refined_batched_dev(const npph_refine_opts* opts, int64
   NPPH_TRY(npph::ensure_init());
   if (!opts || count < 0 || n < 0 || nrhs < 0 || n > npph::max_blocked_n()) {
     npph::set_error("refined solve: bad arguments");
+    return NPPH_ERR_INVALID;
+  }
+  if (B == X) {
+    npph::set_error("refined solve: B and X must not alias");
     return NPPH_ERR_INVALID;
   }
   return npph::refined_impl(*opts, count, (int)n, (int)nrhs, A, B, X, berr, iters, info,

@@ -1,4 +1,5 @@
 // Device probe, error reporting, workspace arena and memory/stream plumbing.
+#include <atomic>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -74,15 +75,51 @@ int ensure_init() {
 hipStream_t default_stream() { return ctx().stream; }
 int cu_count() { return ctx().cus; }
 
+struct WsSet {
+  Workspace ws[kWsCount];
+  hipStream_t owner = nullptr;
+};
+
+namespace {
+std::atomic<uint64_t> g_ws_gen{0};
+thread_local WsSet* t_ws = nullptr;
+}
+
+WsSet* ws_create(hipStream_t owner) {
+  WsSet* w = new WsSet;
+  w->owner = owner;
+  return w;
+}
+
+void ws_destroy(WsSet* w) {
+  if (!w) return;
+  if (w->owner) (void)hipStreamSynchronize(w->owner);
+  for (auto& x : w->ws)
+    if (x.ptr) (void)hipFree(x.ptr);
+  delete w;
+}
+
+void ws_bind(WsSet* w) { t_ws = w; }
+uint64_t workspace_generation() { return g_ws_gen.load(); }
+
+bool stream_capturing(hipStream_t s) {
+  hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(s, &st) != hipSuccess) return false;
+  return st != hipStreamCaptureStatusNone;
+}
+
 int workspace(int slot, size_t bytes, void** out) {
   Context& c = ctx();
   if (slot < 0 || slot >= kWsCount) { set_error("bad workspace slot %d", slot); return NPPH_ERR_INVALID; }
   std::lock_guard<std::mutex> g(c.ws_mu);
-  Workspace& w = c.ws[slot];
+  Workspace& w = t_ws ? t_ws->ws[slot] : c.ws[slot];
   if (w.bytes < bytes) {
+    g_ws_gen.fetch_add(1);
     if (w.ptr) {
-      // The previous buffer may still be in use by queued work on any stream.
-      NPPH_HIP(hipDeviceSynchronize());
+      // The previous buffer may still be in use by queued work: on the owner
+      // stream of a bound set, on any stream for the process-wide set.
+      if (t_ws && t_ws->owner) NPPH_HIP(hipStreamSynchronize(t_ws->owner));
+      else NPPH_HIP(hipDeviceSynchronize());
       NPPH_HIP(hipFree(w.ptr));
       w.ptr = nullptr;
       w.bytes = 0;

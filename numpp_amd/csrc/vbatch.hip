@@ -1,0 +1,445 @@
+// Uneven batches: many independent systems of different order n and nrhs in
+// one call.
+//
+// A plan sorts the systems into size buckets. Each bucket holds a uniform
+// padded stack: a system of order n < N is embedded as diag(A, I) of order N
+// with zero right-hand-side rows, which leaves the leading factorisation, its
+// pivots, its exact-zero test and its solution unchanged (the padded rows and
+// columns never enter the leading block's arithmetic). Buckets then run the
+// uniform kernels: one wavefront per system for N <= 64, the blocked LU with
+// MFMA updates above. The padding granularity per size range is picked by a
+// measured autotuner (per-system time of each candidate bucket order on a
+// calibration stack, cached to a JSON file), so the choice depends only on the
+// system's (n, nrhs) and the precision, never on the rest of the batch.
+//
+// Packed layout (host-visible contract): system i's A starts at element
+// sum_{j<i} n_j^2, its B and X at sum_{j<i} n_j*nrhs_j, its berr at
+// sum_{j<i} nrhs_j; all row-major.
+//
+// The FP64 execution (pack, per-bucket LU + solve, unpack) is captured once into
+// a hipGraph and replayed; the graph is re-captured when the data pointers or a
+// workspace address change.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <fstream>
+#include <map>
+#include <mutex>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+#include "lu.hpp"
+#include "prof.hpp"
+
+namespace npph {
+
+int gesv_dev_impl(int64_t count, int n, int nrhs, double* A, int64_t strideA, double* B, int64_t strideB,
+                  int32_t* ipiv, int32_t* info, hipStream_t s);
+int refined_impl(const npph_refine_opts& o, int64_t count, int n, int nrhs, const double* A, const double* B,
+                 double* X, double* berr, int32_t* iters_out, int32_t* info_out, hipStream_t s);
+
+namespace {
+
+struct Bucket {
+  int N = 0, R = 0;  // padded order, padded nrhs
+  std::vector<int32_t> ids;
+  int32_t* d_ids = nullptr;
+  double* A = nullptr;  // cnt x N x N
+  double* B = nullptr;  // cnt x N x R
+  double* X = nullptr;  // cnt x N x R
+  double* berr = nullptr;
+  int32_t* iters = nullptr;
+  int32_t* info = nullptr;
+  int32_t* ipiv = nullptr;
+};
+
+struct Plan {
+  int64_t count = 0;
+  npph_refine_opts opts{};
+  std::vector<int32_t> n, nrhs;
+  int32_t *d_n = nullptr, *d_nrhs = nullptr;
+  int64_t *d_aoff = nullptr, *d_boff = nullptr, *d_coff = nullptr;
+  std::vector<Bucket> buckets;
+  std::vector<void*> allocs;
+  hipGraphExec_t exec = nullptr;
+  hipGraph_t graph = nullptr;
+  const void* gA = nullptr;
+  const void* gB = nullptr;
+  const void* gX = nullptr;
+  const void* gI = nullptr;
+  const void* gE = nullptr;
+  const void* gT = nullptr;
+  hipStream_t gS = nullptr;
+  uint64_t gGen = ~0ull;
+  bool warmed = false;
+};
+
+// ---- pack / unpack ----
+__global__ __launch_bounds__(256) void pack_kernel(int N, int R, const int32_t* ids, const int32_t* nn,
+                                                   const int32_t* nr, const int64_t* aoff, const int64_t* boff,
+                                                   const double* A, const double* B, double* Ap, double* Bp) {
+  const int64_t k = blockIdx.x;
+  const int id = ids[k];
+  const int n = nn[id], m = nr[id];
+  const double* a = A + aoff[id];
+  const double* b = B + boff[id];
+  double* ap = Ap + k * (int64_t)N * N;
+  double* bp = Bp + k * (int64_t)N * R;
+  for (int e = threadIdx.x; e < N * N; e += 256) {
+    const int i = e / N, j = e - i * N;
+    ap[e] = (i < n && j < n) ? a[(int64_t)i * n + j] : (i == j ? 1.0 : 0.0);
+  }
+  for (int e = threadIdx.x; e < N * R; e += 256) {
+    const int i = e / R, c = e - i * R;
+    bp[e] = (i < n && c < m) ? b[(int64_t)i * m + c] : 0.0;
+  }
+}
+
+__global__ __launch_bounds__(256) void unpack_kernel(int N, int R, const int32_t* ids, const int32_t* nn,
+                                                     const int32_t* nr, const int64_t* boff, const int64_t* coff,
+                                                     const double* Xp, const int32_t* infop, const double* berrp,
+                                                     const int32_t* itersp, double* X, int32_t* info, double* berr,
+                                                     int32_t* iters) {
+  const int64_t k = blockIdx.x;
+  const int id = ids[k];
+  const int n = nn[id], m = nr[id];
+  const double* xp = Xp + k * (int64_t)N * R;
+  double* x = X + boff[id];
+  for (int e = threadIdx.x; e < n * m; e += 256) {
+    const int i = e / m, c = e - i * m;
+    x[e] = xp[(int64_t)i * R + c];
+  }
+  if (threadIdx.x == 0) {
+    if (info) info[id] = infop[k];
+    if (iters) iters[id] = itersp ? itersp[k] : 0;
+  }
+  if (berr && berrp)
+    for (int c = threadIdx.x; c < m; c += 256) berr[coff[id] + c] = berrp[k * (int64_t)R + c];
+}
+
+// ---- autotuner: per-system time of a uniform padded bucket ----
+std::mutex g_tune_mu;
+std::map<std::string, double> g_tune;  // key "prec:N:R" -> microseconds per system
+bool g_tune_loaded = false;
+
+std::string tune_path() {
+  if (const char* e = std::getenv("NUMPP_HIP_TUNE_CACHE")) return e;
+  const char* home = std::getenv("HOME");
+  return std::string(home ? home : "/tmp") + "/.cache/numpp_hip_tune.json";
+}
+
+void tune_load() {
+  if (g_tune_loaded) return;
+  g_tune_loaded = true;
+  std::ifstream f(tune_path());
+  if (!f) return;
+  std::stringstream ss;
+  ss << f.rdbuf();
+  const std::string t = ss.str();
+  size_t p = 0;
+  while ((p = t.find('"', p)) != std::string::npos) {
+    const size_t q = t.find('"', p + 1);
+    if (q == std::string::npos) break;
+    const std::string key = t.substr(p + 1, q - p - 1);
+    const size_t c = t.find(':', q);
+    if (c == std::string::npos) break;
+    g_tune[key] = std::atof(t.c_str() + c + 1);
+    p = t.find_first_of(",}", c);
+    if (p == std::string::npos) break;
+  }
+}
+
+void tune_save() {
+  const std::string path = tune_path();
+  const size_t slash = path.rfind('/');
+  if (slash != std::string::npos) {
+    std::string dir = path.substr(0, slash);
+    std::string cmd = "mkdir -p '" + dir + "' 2>/dev/null";
+    if (std::system(cmd.c_str()) != 0) return;
+  }
+  std::ofstream f(path);
+  if (!f) return;
+  f << "{";
+  bool first = true;
+  for (auto& kv : g_tune) {
+    f << (first ? "" : ",") << "\n  \"" << kv.first << "\": " << kv.second;
+    first = false;
+  }
+  f << "\n}\n";
+}
+
+__global__ void calib_fill_kernel(int64_t total, int N, double* A, double* B, int R) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < total) {
+    const int64_t w = e % ((int64_t)N * N);
+    const int i = (int)(w / N), j = (int)(w % N);
+    // deterministic, diagonally weighted test matrix
+    const uint64_t h = (uint64_t)e * 0x9E3779B97F4A7C15ull;
+    const double u = ((h >> 11) * (1.0 / 9007199254740992.0)) * 2.0 - 1.0;
+    A[e] = u + (i == j ? (double)N : 0.0);
+  }
+  if (e < total / N * R) B[e] = 1.0;
+}
+
+// Measured microseconds per system for a uniform stack of order N.
+int tune_cost(const npph_refine_opts& o, int N, int R, double* us, hipStream_t s) {
+  char key[64];
+  std::snprintf(key, sizeof key, "p%d:%d:%d", o.precision, N, R);
+  {
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    tune_load();
+    auto it = g_tune.find(key);
+    if (it != g_tune.end()) { *us = it->second; return NPPH_OK; }
+  }
+  const int64_t cnt = N <= 64 ? 1024 : (N <= 128 ? 256 : 128);
+  double *A, *A0, *B, *X, *E;
+  int32_t *I, *F;
+  NPPH_HIP(hipMalloc(&A, sizeof(double) * cnt * N * N));
+  NPPH_HIP(hipMalloc(&A0, sizeof(double) * cnt * N * N));
+  NPPH_HIP(hipMalloc(&B, sizeof(double) * cnt * N * R));
+  NPPH_HIP(hipMalloc(&X, sizeof(double) * cnt * N * R));
+  NPPH_HIP(hipMalloc(&E, sizeof(double) * cnt * R));
+  NPPH_HIP(hipMalloc(&I, sizeof(int32_t) * cnt));
+  NPPH_HIP(hipMalloc(&F, sizeof(int32_t) * cnt));
+  const int64_t total = cnt * N * N;
+  hipLaunchKernelGGL(calib_fill_kernel, dim3((unsigned)ceil_div(total, 256)), dim3(256), 0, s, total, N, A0, B, R);
+  hipEvent_t e0, e1;
+  NPPH_HIP(hipEventCreate(&e0));
+  NPPH_HIP(hipEventCreate(&e1));
+  float best = 1e30f;
+  int st = NPPH_OK;
+  for (int rep = 0; rep < 3 && st == NPPH_OK; ++rep) {
+    NPPH_HIP(hipMemcpyAsync(A, A0, sizeof(double) * total, hipMemcpyDeviceToDevice, s));
+    NPPH_HIP(hipEventRecord(e0, s));
+    if (o.precision == NPPH_PREC_FP64) {
+      st = gesv_dev_impl(cnt, N, R, A, (int64_t)N * N, B, (int64_t)N * R, nullptr, nullptr, s);
+    } else {
+      st = refined_impl(o, cnt, N, R, A, B, X, E, I, F, s);
+    }
+    NPPH_HIP(hipEventRecord(e1, s));
+    NPPH_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    NPPH_HIP(hipEventElapsedTime(&ms, e0, e1));
+    if (rep > 0) best = std::min(best, ms);
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(A); (void)hipFree(A0); (void)hipFree(B); (void)hipFree(X); (void)hipFree(E);
+  (void)hipFree(I); (void)hipFree(F);
+  if (st != NPPH_OK) return st;
+  *us = 1e3 * best / cnt;
+  std::lock_guard<std::mutex> g(g_tune_mu);
+  g_tune[key] = *us;
+  tune_save();
+  return NPPH_OK;
+}
+
+int round_up(int v, int g) { return (v + g - 1) / g * g; }
+int small_order(int n) { return n <= 8 ? 8 : n <= 16 ? 16 : n <= 32 ? 32 : 64; }
+
+// Bucket order for a system of order n under padding granularity G (n > 64).
+int bucket_order(int n, int G) { return n <= 64 ? small_order(n) : round_up(n, G); }
+
+template <class T>
+int dalloc(Plan* p, T** out, size_t count) {
+  void* q = nullptr;
+  NPPH_HIP(hipMalloc(&q, sizeof(T) * (count ? count : 1)));
+  p->allocs.push_back(q);
+  *out = static_cast<T*>(q);
+  return NPPH_OK;
+}
+
+int plan_build(Plan* p, int autotune, hipStream_t s) {
+  const int64_t count = p->count;
+  int maxr = 1;
+  for (int64_t i = 0; i < count; ++i) maxr = std::max(maxr, (int)p->nrhs[i]);
+  const int R = maxr <= 8 ? std::max(1, maxr) : round_up(maxr, 8);
+  // Candidate padding granularities for n > 64; pick the one with the least
+  // measured total time (sum over buckets of count x per-system time).
+  std::vector<int> cands = {32, 64};
+  int bestG = 32;
+  if (autotune) {
+    double bestT = 1e300;
+    for (int G : cands) {
+      std::map<int, int64_t> hist;
+      for (int64_t i = 0; i < count; ++i) hist[bucket_order(p->n[i], G)]++;
+      double tot = 0.0;
+      for (auto& kv : hist) {
+        double us = 0.0;
+        NPPH_TRY(tune_cost(p->opts, kv.first, R, &us, s));
+        tot += us * kv.second;
+      }
+      if (tot < bestT) { bestT = tot; bestG = G; }
+    }
+  }
+  std::map<int, std::vector<int32_t>> by;
+  for (int64_t i = 0; i < count; ++i) by[bucket_order(p->n[i], bestG)].push_back((int32_t)i);
+  for (auto& kv : by) {
+    Bucket b;
+    b.N = kv.first;
+    b.R = R;
+    b.ids = kv.second;
+    const size_t c = b.ids.size();
+    NPPH_TRY(dalloc(p, &b.d_ids, c));
+    NPPH_TRY(dalloc(p, &b.A, c * b.N * b.N));
+    NPPH_TRY(dalloc(p, &b.B, c * b.N * b.R));
+    NPPH_TRY(dalloc(p, &b.X, c * b.N * b.R));
+    NPPH_TRY(dalloc(p, &b.berr, c * b.R));
+    NPPH_TRY(dalloc(p, &b.iters, c));
+    NPPH_TRY(dalloc(p, &b.info, c));
+    NPPH_TRY(dalloc(p, &b.ipiv, c * b.N));
+    NPPH_HIP(hipMemcpyAsync(b.d_ids, b.ids.data(), sizeof(int32_t) * c, hipMemcpyHostToDevice, s));
+    p->buckets.push_back(std::move(b));
+  }
+  NPPH_HIP(hipStreamSynchronize(s));
+  return NPPH_OK;
+}
+
+// One execution without graph capture.
+int run_plan(Plan* p, const double* A, const double* B, double* X, double* berr, int32_t* iters, int32_t* info,
+             hipStream_t s) {
+  const bool fp64 = p->opts.precision == NPPH_PREC_FP64;
+  for (auto& b : p->buckets) {
+    const int64_t c = (int64_t)b.ids.size();
+    hipLaunchKernelGGL(pack_kernel, dim3((unsigned)c), dim3(256), 0, s, b.N, b.R, b.d_ids, p->d_n, p->d_nrhs,
+                       p->d_aoff, p->d_boff, A, B, b.A, fp64 ? b.X : b.B);
+    NPPH_HIP(hipGetLastError());
+    if (fp64) {
+      NPPH_TRY(gesv_dev_impl(c, b.N, b.R, b.A, (int64_t)b.N * b.N, b.X, (int64_t)b.N * b.R, b.ipiv, b.info, s));
+    } else {
+      NPPH_TRY(refined_impl(p->opts, c, b.N, b.R, b.A, b.B, b.X, b.berr, b.iters, b.info, s));
+    }
+    hipLaunchKernelGGL(unpack_kernel, dim3((unsigned)c), dim3(256), 0, s, b.N, b.R, b.d_ids, p->d_n, p->d_nrhs,
+                       p->d_boff, p->d_coff, b.X, b.info, fp64 ? nullptr : b.berr, fp64 ? nullptr : b.iters, X, info,
+                       berr, iters);
+    NPPH_HIP(hipGetLastError());
+  }
+  return NPPH_OK;
+}
+
+}  // namespace
+
+}  // namespace npph
+
+using namespace npph;
+
+extern "C" {
+
+int npph_vbatch_plan_create(int64_t count, const int32_t* n, const int32_t* nrhs, const npph_refine_opts* opts,
+                            int autotune, void** plan_out) {
+  NPPH_TRY(ensure_init());
+  if (count <= 0 || !n || !nrhs || !opts || !plan_out) { set_error("vbatch: bad arguments"); return NPPH_ERR_INVALID; }
+  if (count > 65535 * 16) { set_error("vbatch: count too large"); return NPPH_ERR_INVALID; }
+  for (int64_t i = 0; i < count; ++i) {
+    if (n[i] < 1 || n[i] > max_blocked_n() || nrhs[i] < 1 || nrhs[i] > 4096) {
+      set_error("vbatch: system %ld has n=%d nrhs=%d", (long)i, n[i], nrhs[i]);
+      return NPPH_ERR_INVALID;
+    }
+  }
+  Plan* p = new Plan;
+  p->count = count;
+  p->opts = *opts;
+  p->n.assign(n, n + count);
+  p->nrhs.assign(nrhs, nrhs + count);
+  std::vector<int64_t> aoff(count), boff(count), coff(count);
+  int64_t a = 0, b = 0, c = 0;
+  for (int64_t i = 0; i < count; ++i) {
+    aoff[i] = a; boff[i] = b; coff[i] = c;
+    a += (int64_t)n[i] * n[i];
+    b += (int64_t)n[i] * nrhs[i];
+    c += nrhs[i];
+  }
+  hipStream_t s = default_stream();
+  int st = NPPH_OK;
+  auto up = [&](auto** dst, const auto* src) {
+    if (st != NPPH_OK) return;
+    st = dalloc(p, dst, count);
+    if (st == NPPH_OK && hipMemcpyAsync(*dst, src, sizeof(**dst) * count, hipMemcpyHostToDevice, s) != hipSuccess) {
+      set_error("vbatch: upload failed");
+      st = NPPH_ERR_HIP;
+    }
+  };
+  up(&p->d_n, n);
+  up(&p->d_nrhs, nrhs);
+  up(&p->d_aoff, aoff.data());
+  up(&p->d_boff, boff.data());
+  up(&p->d_coff, coff.data());
+  if (st == NPPH_OK) st = plan_build(p, autotune, s);
+  if (st != NPPH_OK) {
+    for (void* q : p->allocs) (void)hipFree(q);
+    delete p;
+    return st;
+  }
+  *plan_out = p;
+  return NPPH_OK;
+}
+
+int npph_vbatch_plan_destroy(void* plan) {
+  NPPH_TRY(ensure_init());
+  Plan* p = static_cast<Plan*>(plan);
+  if (!p) return NPPH_OK;
+  NPPH_HIP(hipDeviceSynchronize());
+  if (p->exec) (void)hipGraphExecDestroy(p->exec);
+  if (p->graph) (void)hipGraphDestroy(p->graph);
+  for (void* q : p->allocs) (void)hipFree(q);
+  delete p;
+  return NPPH_OK;
+}
+
+// Bucket summary: nbuckets on return; order/count/padded-rhs arrays filled up
+// to `cap` entries when non-null.
+int npph_vbatch_plan_info(void* plan, int* nbuckets, int32_t* order, int64_t* counts, int32_t* rpad, int cap) {
+  Plan* p = static_cast<Plan*>(plan);
+  if (!p || !nbuckets) { set_error("vbatch: bad plan"); return NPPH_ERR_INVALID; }
+  *nbuckets = (int)p->buckets.size();
+  for (int i = 0; i < (int)p->buckets.size() && i < cap; ++i) {
+    if (order) order[i] = p->buckets[i].N;
+    if (counts) counts[i] = (int64_t)p->buckets[i].ids.size();
+    if (rpad) rpad[i] = p->buckets[i].R;
+  }
+  return NPPH_OK;
+}
+
+int npph_vbatch_execute(void* plan, const double* A, const double* B, double* X, double* berr, int32_t* iters,
+                        int32_t* info, int use_graph, void* stream) {
+  NPPH_TRY(ensure_init());
+  Plan* p = static_cast<Plan*>(plan);
+  if (!p || !A || !B || !X) { set_error("vbatch: bad arguments"); return NPPH_ERR_INVALID; }
+  hipStream_t s = stream ? (hipStream_t)stream : default_stream();
+  const bool graphable = use_graph && p->opts.precision == NPPH_PREC_FP64;
+  if (!graphable) return run_plan(p, A, B, X, berr, iters, info, s);
+  // First call: run eagerly so every workspace slot reaches its final size.
+  if (!p->warmed) {
+    NPPH_TRY(run_plan(p, A, B, X, berr, iters, info, s));
+    p->warmed = true;
+    return NPPH_OK;
+  }
+  const bool stale = !p->exec || p->gA != A || p->gB != B || p->gX != X || p->gI != info || p->gE != berr || p->gT != iters || p->gS != s ||
+                     p->gGen != workspace_generation();
+  if (stale) {
+    if (p->exec) { (void)hipGraphExecDestroy(p->exec); p->exec = nullptr; }
+    if (p->graph) { (void)hipGraphDestroy(p->graph); p->graph = nullptr; }
+    const uint64_t gen = workspace_generation();
+    NPPH_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    const int st = run_plan(p, A, B, X, berr, iters, info, s);
+    hipGraph_t g = nullptr;
+    const hipError_t ec = hipStreamEndCapture(s, &g);
+    if (st != NPPH_OK) { if (g) (void)hipGraphDestroy(g); return st; }
+    NPPH_HIP(ec);
+    if (workspace_generation() != gen) {
+      (void)hipGraphDestroy(g);
+      set_error("vbatch: workspace grew during capture");
+      return NPPH_ERR_INVALID;
+    }
+    p->graph = g;
+    NPPH_HIP(hipGraphInstantiate(&p->exec, g, nullptr, nullptr, 0));
+    p->gA = A; p->gB = B; p->gX = X; p->gI = info; p->gE = berr; p->gT = iters; p->gS = s; p->gGen = gen;
+  }
+  NPPH_HIP(hipGraphLaunch(p->exec, s));
+  return NPPH_OK;
+}
+
+}  // extern "C"
