@@ -2,7 +2,8 @@
 //
 // FP64 uses v_mfma_f64_16x16x4f64, FP32 uses v_mfma_f32_16x16x4f32; both take
 // one A element and one B element per lane (A[i][k]: lane = i + 16k, B[k][j]:
-// lane = j + 16k) and return four C elements per lane (C[4*(lane/16)+v][lane%16]).
+// lane = j + 16k) and return four C elements per lane, column lane%16, rows
+// lane/16 + 4v for the FP64 instruction and 4*(lane/16) + v for the FP32 one.
 //
 // Workgroup: 256 threads = 4 waves in a 2 x 2 grid; macro tile BM x BN = 128 x 128,
 // each wave owns a 64 x 64 sub-tile (4 x 4 MFMA tiles, 16 accumulators).
@@ -32,6 +33,7 @@ struct Mfma;
 template <>
 struct Mfma<double> {
   typedef v4d acc_t;
+  static __device__ __forceinline__ int row(int lk, int v) { return lk + 4 * v; }
   static __device__ __forceinline__ acc_t run(double a, double b, acc_t c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
@@ -39,6 +41,7 @@ struct Mfma<double> {
 template <>
 struct Mfma<float> {
   typedef v4f acc_t;
+  static __device__ __forceinline__ int row(int lk, int v) { return 4 * lk + v; }
   static __device__ __forceinline__ acc_t run(float a, float b, acc_t c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
@@ -156,7 +159,7 @@ __global__ __launch_bounds__(NT) void gemm_kernel(GemmArgs<T> p) {
   for (int i = 0; i < 4; ++i) {
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
-      const int64_t row = m0 + wm * 64 + i * 16 + 4 * lk + v;
+      const int64_t row = m0 + wm * 64 + i * 16 + Mfma<T>::row(lk, v);
       if (row >= p.M) continue;
       T* crow = C + row * p.ldc;
 #pragma unroll
