@@ -95,7 +95,7 @@ int npph_sgemm_dev(int64_t m, int64_t n, int64_t k, float alpha, const float* A,
                    int64_t ldc, int64_t sC, int64_t batch, void* stream);
 
 /* Device-resident C -= A B for FP32 matrices through the split-BF16 MFMA
- * update of the low-precision LU (terms 1, 3 or 6; K <= 64). No reference
+ * update of the low-precision LU (terms 1, 3 or 6). No reference
  * counterpart; exposed so the update can be checked on its own. */
 int npph_bf16split_update_dev(int64_t M, int64_t N, int64_t K, const float* A, int64_t lda,
                               const float* B, int64_t ldb, float* C, int64_t ldc, int terms,
