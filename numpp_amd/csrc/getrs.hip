@@ -4,12 +4,13 @@
 // row permutation, forward substitution with the unit-lower L, back
 // substitution with U dividing by its diagonal.
 //
-// Each sweep is one launch with one 256-thread workgroup per (64-row block,
-// matrix, 8-column RHS chunk). Block i first accumulates the contributions of
+// Each sweep is one launch with one 256-thread workgroup per (matrix, 64-row
+// block, 8-column RHS chunk). Block i first accumulates the contributions of
 // the blocks it depends on (L_ij X_j for j < i forward, U_ij X_j for j > i
 // backward) as soon as each X_j is published, reading its 64-row strip of the
 // factor coalesced, then solves its 64 x 64 diagonal block with one wave
-// (lane = row, v_readlane broadcasts), publishes X_i and raises its flag
+// (lane = row, v_readlane broadcasts; U's diagonal is applied as a multiply by
+// its reciprocal), publishes X_i and raises its flag
 // (release at device scope; readers acquire). Workgroups are dispatched in
 // index order and a block only waits for lower-indexed ones, so every wait
 // ends; a bounded spin reports an error instead of hanging.
@@ -49,11 +50,13 @@ __global__ __launch_bounds__(GT) void sweep_kernel(int n, int nrhs, const T* LUb
                                                    int64_t strideX, int* flags, int* err, const int32_t* info,
                                                    const int32_t* mask) {
   const int nblk = (n + GB - 1) / GB;
-  const int64_t mat = blockIdx.y;
+  // x = matrix (fastest), y = block: the resident workgroups cover the early
+  // blocks of every matrix, so all chains advance together.
+  const int64_t mat = blockIdx.x;
   const int z = blockIdx.z;
   int* fl = flags + (mat * gridDim.z + z) * (int64_t)nblk;
   if ((info && info[mat] != 0) || (mask && mask[mat] == 0)) return;
-  const int bi = LOWER ? (int)blockIdx.x : nblk - 1 - (int)blockIdx.x;
+  const int bi = LOWER ? (int)blockIdx.y : nblk - 1 - (int)blockIdx.y;
   const int r0 = bi * GB, bs = min(GB, n - r0);
   const int c0 = z * GNR, nc = min(GNR, nrhs - c0);
   const T* __restrict__ LU = LUb + mat * strideA;
@@ -132,11 +135,14 @@ __global__ __launch_bounds__(GT) void sweep_kernel(int n, int nrhs, const T* LUb
       T d = T(1);
 #pragma unroll
       for (int q = 0; q < GB; ++q) d = (q == lane && lane < bs) ? row[q] : d;
+      // one division per lane, in parallel; the row's scaling at its step is
+      // then a multiply (within one rounding of lu_solve's division)
+      const T dinv = T(1) / d;
 #pragma unroll
       for (int q = GB - 1; q >= 0; --q) {
         if (lane == q) {
 #pragma unroll
-          for (int c = 0; c < GNR; ++c) x[c] = x[c] / d;
+          for (int c = 0; c < GNR; ++c) x[c] = x[c] * dinv;
         }
         const T f = lane < q ? row[q] : T(0);
 #pragma unroll
@@ -196,7 +202,7 @@ int blocked_getrs_perm(int64_t count, int n, int nrhs, const T* LU, int64_t stri
   int* err = flags + 2 * nflag;
   hipLaunchKernelGGL(clear_flags_kernel, dim3((unsigned)ceil_div(2 * nflag + 1, 256)), dim3(256), 0, s, flags,
                      2 * nflag + 1);
-  dim3 grid((unsigned)nblk, (unsigned)count, (unsigned)nz);
+  dim3 grid((unsigned)count, (unsigned)nblk, (unsigned)nz);
   ProfScope ps(kProfGetrs, 2.0 * count * n * n * nrhs,
                (double)count * (sizeof(T) * (double)n * n + 3.0 * sizeof(T) * n * nrhs), s);
   hipLaunchKernelGGL((sweep_kernel<T, true>), grid, dim3(GT), 0, s, n, nrhs, LU, strideA, perm, B, strideB, X,

@@ -68,7 +68,7 @@ __global__ __launch_bounds__(PT) void leaf_kernel(LeafArgs<T> a) {
   int32_t* ipiv = a.ipiv + mat * n;
   int32_t* perm = a.perm + mat * n;
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-  constexpr int QC = R >= 8 ? 4 : 8;  // L columns per update chunk
+  constexpr int QC = (R >= 4 && sizeof(T) == 8) ? 4 : 8;  // L columns per update chunk
 
   __shared__ T L11[LEAF][LEAF + 1];
   __shared__ T Ublk[LEAF][IB];
@@ -118,24 +118,37 @@ __global__ __launch_bounds__(PT) void leaf_kernel(LeafArgs<T> a) {
       }
     }
     __syncthreads();
-    // ---- left-looking update with the L columns [k0, cb) of each row
-    for (int q0 = 0; q0 < jb; q0 += QC) {
-      T lc[R][QC];
+    // ---- left-looking update with the L columns [k0, cb) of each row, in
+    // chunks of QC columns; the next chunk's loads are in flight while the
+    // current one is applied
+    if (jb > 0) {
+      auto loadc = [&](T (&dst)[R][QC], int q0) {
 #pragma unroll
-      for (int s = 0; s < R; ++s) {
-        const T* lr = A + (int64_t)(k0 + (pos[s] < m ? pos[s] : 0)) * n + k0 + q0;
+        for (int s = 0; s < R; ++s) {
+          const T* lr = A + (int64_t)(k0 + (pos[s] < m ? pos[s] : 0)) * n + k0 + q0;
 #pragma unroll
-        for (int q = 0; q < QC; ++q) lc[s][q] = (pos[s] < m && q0 + q < jb) ? lr[q] : T(0);
-      }
-#pragma unroll
-      for (int q = 0; q < QC; ++q) {
-        if (q0 + q >= jb) break;
-#pragma unroll
-        for (int c = 0; c < IB; ++c) {
-          const T u = Ublk[q0 + q][c];
-#pragma unroll
-          for (int s = 0; s < R; ++s) v[s][c] = v[s][c] - lc[s][q] * u;
+          for (int q = 0; q < QC; ++q) dst[s][q] = (pos[s] < m && q0 + q < jb) ? lr[q] : T(0);
         }
+      };
+      auto applyc = [&](const T (&src)[R][QC], int q0) {
+#pragma unroll
+        for (int q = 0; q < QC; ++q) {
+          if (q0 + q >= jb) break;
+#pragma unroll
+          for (int c = 0; c < IB; ++c) {
+            const T u = Ublk[q0 + q][c];
+#pragma unroll
+            for (int s = 0; s < R; ++s) v[s][c] = v[s][c] - src[s][q] * u;
+          }
+        }
+      };
+      T la[R][QC], lb[R][QC];
+      loadc(la, 0);
+      for (int q0 = 0; q0 < jb; q0 += 2 * QC) {
+        if (q0 + QC < jb) loadc(lb, q0 + QC);
+        applyc(la, q0);
+        if (q0 + 2 * QC < jb) loadc(la, q0 + 2 * QC);
+        if (q0 + QC < jb) applyc(lb, q0 + QC);
       }
     }
     // ---- factor the block's columns, one barrier per column

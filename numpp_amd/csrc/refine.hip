@@ -37,10 +37,18 @@ __global__ __launch_bounds__(256) void demote_kernel(int64_t count, int n, const
   const double* a = A + gw * n;
   float* o = Alo + gw * n;
   double s = 0.0;
-  for (int j = lane; j < n; j += 64) {
-    const double v = a[j];
-    o[j] = (float)v;
-    s += fabs(v);
+  if ((n & 1) == 0) {
+    for (int j = 2 * lane; j < n; j += 128) {
+      const double2 v = *reinterpret_cast<const double2*>(a + j);
+      *reinterpret_cast<float2*>(o + j) = make_float2((float)v.x, (float)v.y);
+      s += fabs(v.x) + fabs(v.y);
+    }
+  } else {
+    for (int j = lane; j < n; j += 64) {
+      const double v = a[j];
+      o[j] = (float)v;
+      s += fabs(v);
+    }
   }
 #pragma unroll
   for (int m = 32; m >= 1; m >>= 1) s += __shfl_xor(s, m, 64);
@@ -63,7 +71,14 @@ __global__ __launch_bounds__(256) void rowabs_kernel(int64_t count, int n, const
   if (gw >= count * n) return;
   const double* a = A + gw * n;
   double s = 0.0;
-  for (int j = lane; j < n; j += 64) s += fabs(a[j]);
+  if ((n & 1) == 0) {
+    for (int j = 2 * lane; j < n; j += 128) {
+      const double2 v = *reinterpret_cast<const double2*>(a + j);
+      s += fabs(v.x) + fabs(v.y);
+    }
+  } else {
+    for (int j = lane; j < n; j += 64) s += fabs(a[j]);
+  }
 #pragma unroll
   for (int m = 32; m >= 1; m >>= 1) s += __shfl_xor(s, m, 64);
   if (lane == 0) atomic_max_nonneg(anorm + gw / n, s);
@@ -131,7 +146,7 @@ __global__ void count_sweep_kernel(int64_t count, const int32_t* active, int32_t
 // each lane owns two adjacent A columns per chunk (one 16-byte load per row).
 // The 64 partial sums per lane (8 rows x 8 columns) are combined with a
 // halving butterfly, after which lane L owns (row L/8, column L%8).
-constexpr int RES_ROWS = 32, RES_JC = 128, RES_NC = 8;
+constexpr int RES_ROWS = 32, RES_JC = 512, RES_NC = 8;
 __global__ __launch_bounds__(256) void residual_kernel(int n, int nrhs, const double* Ab, const double* Xb,
                                                        const double* Bb, double* Rb, double* rmax, double* xmax,
                                                        const int32_t* active) {
@@ -163,26 +178,34 @@ __global__ __launch_bounds__(256) void residual_kernel(int n, int nrhs, const do
       xm = fmax(xm, fabs(v));
     }
     __syncthreads();
-    const int j0 = jc + 2 * lane;
-    double x0[RES_NC], x1[RES_NC];
 #pragma unroll
-    for (int c = 0; c < RES_NC; ++c) { x0[c] = xs[2 * lane][c]; x1[c] = xs[2 * lane + 1][c]; }
+    for (int sub = 0; sub < RES_JC / 128; ++sub) {
+      const int jl = sub * 128 + 2 * lane;
+      const int j0 = jc + jl;
+      if (j0 >= n) break;
+      double a0[8], a1[8];
 #pragma unroll
-    for (int r = 0; r < 8; ++r) {
-      const int row = row0 + r;
-      double a0 = 0.0, a1 = 0.0;
-      if (row < n) {
-        const double* ar = A + (int64_t)row * n;
-        if (even && j0 + 1 < n) {
-          const double2 v = *reinterpret_cast<const double2*>(ar + j0);
-          a0 = v.x; a1 = v.y;
-        } else {
-          a0 = j0 < n ? ar[j0] : 0.0;
-          a1 = j0 + 1 < n ? ar[j0 + 1] : 0.0;
+      for (int r = 0; r < 8; ++r) {
+        const int row = row0 + r;
+        a0[r] = 0.0;
+        a1[r] = 0.0;
+        if (row < n) {
+          const double* ar = A + (int64_t)row * n;
+          if (even && j0 + 1 < n) {
+            const double2 v = *reinterpret_cast<const double2*>(ar + j0);
+            a0[r] = v.x; a1[r] = v.y;
+          } else {
+            a0[r] = ar[j0];
+            a1[r] = j0 + 1 < n ? ar[j0 + 1] : 0.0;
+          }
         }
       }
 #pragma unroll
-      for (int c = 0; c < RES_NC; ++c) acc[r][c] = fma(a1, x1[c], fma(a0, x0[c], acc[r][c]));
+      for (int c = 0; c < RES_NC; ++c) {
+        const double x0 = xs[jl][c], x1 = xs[jl + 1][c];
+#pragma unroll
+        for (int r = 0; r < 8; ++r) acc[r][c] = fma(a1[r], x1, fma(a0[r], x0, acc[r][c]));
+      }
     }
     __syncthreads();
   }
