@@ -346,7 +346,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="mixed", choices=["mixed", "gemm", "plain64"])
     ap.add_argument("--count", type=int, default=64)
     ap.add_argument("--n", type=int, default=2048)

@@ -31,13 +31,16 @@ constexpr unsigned kSpinLimit = 1u << 22;
 // number of polls or as soon as another block has given up.
 __device__ __forceinline__ bool wait_flag(const int* f, int* err) {
   unsigned spins = 0;
-  while (__hip_atomic_load(f, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == 0) {
-    __builtin_amdgcn_s_sleep(1);
+  // Poll relaxed (an acquire per poll would invalidate caches on every spin),
+  // then acquire once the flag is seen.
+  while (__hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) {
+    __builtin_amdgcn_s_sleep(2);
     if (++spins > kSpinLimit || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
       atomicExch(err, 1);
       return false;
     }
   }
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
   return true;
 }
 

@@ -15,7 +15,7 @@
 // and the trailing matrix is read O(log n) times instead of O(n/64).
 //
 // The leaf factors an (n-k0) x w panel with one 512-thread workgroup per
-// matrix, left-looking in inner blocks of 8 columns: the block's columns are
+// matrix, left-looking in inner blocks of 8 to 32 columns: the block's columns are
 // gathered through the leaf's row permutation (rows are not moved until the
 // block is done), brought up to date with the block's U rows (one wave per
 // column solves with L11 via v_readlane) and the L columns already computed,
@@ -28,6 +28,9 @@
 // The rank-1 updates of linalg.cpp:64-68 are regrouped into these blocks; each
 // entry receives the same terms in a different association order.
 #include <algorithm>
+#include <cstdlib>
+#include <mutex>
+#include <vector>
 
 #include "common.hpp"
 #include "gemm.hpp"
@@ -39,7 +42,6 @@ namespace npph {
 namespace {
 
 constexpr int LEAF = 64;   // leaf panel width
-constexpr int IB = 8;      // inner register block width
 constexpr int PT = 512;    // leaf threads
 constexpr int NW = PT / 64;
 constexpr int MVLEAF = 2 * LEAF;  // max net moves of one leaf
@@ -58,7 +60,7 @@ struct LeafArgs {
   const int32_t* mask;
 };
 
-template <class T, int R>
+template <class T, int R, int IB>
 __global__ __launch_bounds__(PT) void leaf_kernel(LeafArgs<T> a) {
   const int64_t mat = blockIdx.x;
   if (a.mask && a.mask[mat] == 0) return;
@@ -105,16 +107,19 @@ __global__ __launch_bounds__(PT) void leaf_kernel(LeafArgs<T> a) {
       Ublk[q][c] = c < ib ? A[(int64_t)(k0 + pid[q]) * n + cb + c] : T(0);
     }
     __syncthreads();
-    // ---- U rows [0, jb) of the block: wave c solves column c (lane = row)
-    if (jb > 0 && wave < ib) {
-      T x = lane < jb ? Ublk[lane][wave] : T(0);
-      for (int p = 0; p < jb - 1; ++p) {
-        const T xp = readlane<T>(x, p);
-        if (lane > p && lane < jb) x = x - L11[lane][p] * xp;
-      }
-      if (lane < jb) {
-        Ublk[lane][wave] = x;
-        A[(int64_t)(k0 + lane) * n + cb + wave] = x;
+    // ---- U rows [0, jb) of the block: wave w solves columns w, w+8, ...
+    // (lane = row)
+    if (jb > 0) {
+      for (int cc = wave; cc < ib; cc += NW) {
+        T x = lane < jb ? Ublk[lane][cc] : T(0);
+        for (int p = 0; p < jb - 1; ++p) {
+          const T xp = readlane<T>(x, p);
+          if (lane > p && lane < jb) x = x - L11[lane][p] * xp;
+        }
+        if (lane < jb) {
+          Ublk[lane][cc] = x;
+          A[(int64_t)(k0 + lane) * n + cb + cc] = x;
+        }
       }
     }
     __syncthreads();
@@ -151,13 +156,14 @@ __global__ __launch_bounds__(PT) void leaf_kernel(LeafArgs<T> a) {
         if (q0 + QC < jb) applyc(lb, q0 + QC);
       }
     }
-    // ---- factor the block's columns, one barrier per column
+    // ---- factor the block's columns, one barrier per column. The column
+    // index is a runtime value; register rows are read and written through
+    // select chains over compile-time indices so they stay in registers.
     int opos[R];
 #pragma unroll
     for (int s = 0; s < R; ++s) opos[s] = pos[s];
-#pragma unroll
-    for (int c = 0; c < IB; ++c) {
-      if (c >= ib) break;
+#pragma unroll 1
+    for (int c = 0; c < ib; ++c) {
       const int col = jb + c;  // current leaf-relative step
       const int buf = c & 1;
       double bv = -1.0;
@@ -165,7 +171,10 @@ __global__ __launch_bounds__(PT) void leaf_kernel(LeafArgs<T> a) {
 #pragma unroll
       for (int s = 0; s < R; ++s) {
         if (pos[s] >= col && pos[s] < m) {
-          const double av = fabs((double)v[s][c]);
+          T vc = v[s][0];
+#pragma unroll
+          for (int j = 1; j < IB; ++j) vc = (j == c) ? v[s][j] : vc;
+          const double av = fabs((double)vc);
           if (piv_better(av, pos[s], bv, bp)) { bv = av; bp = pos[s]; bs = s; }
         }
       }
@@ -202,35 +211,46 @@ __global__ __launch_bounds__(PT) void leaf_kernel(LeafArgs<T> a) {
       T prow[IB];
 #pragma unroll
       for (int j = 0; j < IB; ++j) prow[j] = red_row[buf][wbest][j];
+      T piv = prow[0];
+#pragma unroll
+      for (int j = 1; j < IB; ++j) piv = (j == c) ? prow[j] : piv;
 #pragma unroll
       for (int s = 0; s < R; ++s) {
         if (pos[s] == p) pos[s] = col;
         else if (pos[s] == col) pos[s] = p;
       }
-      const T piv = prow[c];
 #pragma unroll
       for (int s = 0; s < R; ++s) {
         if (pos[s] > col && pos[s] < m) {
-          const T l = piv != T(0) ? v[s][c] / piv : T(0);
-          v[s][c] = l;
+          T vc = v[s][0];
 #pragma unroll
-          for (int j = 0; j < IB; ++j)
-            if (j > c) v[s][j] = v[s][j] - l * prow[j];
+          for (int j = 1; j < IB; ++j) vc = (j == c) ? v[s][j] : vc;
+          const T l = piv != T(0) ? vc / piv : T(0);
+#pragma unroll
+          for (int j = 0; j < IB; ++j) {
+            const T upd = v[s][j] - l * prow[j];
+            v[s][j] = (j == c) ? l : ((j > c) ? upd : v[s][j]);
+          }
         }
       }
     }
-    // ---- L11 rows of the block's new pivot rows (their left columns come
-    // from their pre-block places), block columns from registers
+    // ---- L11 rows of the block's new pivot rows: block columns from
+    // registers, left columns (from their pre-block places) loaded by all
+    // threads in parallel
     if (t == 0) mv_cnt = 0;
 #pragma unroll
     for (int s = 0; s < R; ++s) {
       if (pos[s] >= jb && pos[s] < jb + ib) {
-        const T* lr = A + (int64_t)(k0 + opos[s]) * n + k0;
-        for (int q = 0; q < jb; ++q) L11[pos[s]][q] = lr[q];
+        stage_pid[pos[s] - jb] = opos[s];
 #pragma unroll
         for (int c = 0; c < IB; ++c)
           if (c < ib) L11[pos[s]][jb + c] = v[s][c];
       }
+    }
+    __syncthreads();
+    for (int e = t; e < ib * jb; e += PT) {
+      const int rr = e / jb, q = e - rr * jb;
+      L11[jb + rr][q] = A[(int64_t)(k0 + stage_pid[rr]) * n + k0 + q];
     }
     __syncthreads();
     // ---- write the block at final places; list rows whose place changed
@@ -338,10 +358,19 @@ __global__ __launch_bounds__(256) void move_rows_kernel(T* Abase, int64_t stride
   T* A = Abase + mat * strideA;
   T* sc = scratch + mat * scr_stride;
   const int e1 = min(cnt, e0 + 16);
+  T buf[16];
   if (phase == 0) {
-    for (int e = e0; e < e1; ++e) sc[(int64_t)e * nc + c] = A[(int64_t)mv[1 + 2 * e] * n + c0 + c];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) buf[q] = e0 + q < e1 ? A[(int64_t)mv[1 + 2 * (e0 + q)] * n + c0 + c] : T(0);
+#pragma unroll
+    for (int q = 0; q < 16; ++q)
+      if (e0 + q < e1) sc[(int64_t)(e0 + q) * nc + c] = buf[q];
   } else {
-    for (int e = e0; e < e1; ++e) A[(int64_t)mv[2 + 2 * e] * n + c0 + c] = sc[(int64_t)e * nc + c];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) buf[q] = e0 + q < e1 ? sc[(int64_t)(e0 + q) * nc + c] : T(0);
+#pragma unroll
+    for (int q = 0; q < 16; ++q)
+      if (e0 + q < e1) A[(int64_t)mv[2 + 2 * (e0 + q)] * n + c0 + c] = buf[q];
   }
 }
 
@@ -354,9 +383,14 @@ __global__ __launch_bounds__(64) void trsm_leaf_kernel(T* Abase, int64_t strideA
   if (mask && mask[mat] == 0) return;
   T* A = Abase + mat * strideA;
   __shared__ T L[LEAF][LEAF + 1];
-  for (int e = threadIdx.x; e < s * s; e += 64) {
-    const int i = e / s, j = e - i * s;
-    L[i][j] = A[(int64_t)(r0 + i) * n + r0 + j];
+  {
+    // thread j loads column j of L: all 64 loads in flight before the LDS writes
+    const int j = threadIdx.x;
+    T colv[LEAF];
+#pragma unroll
+    for (int i = 0; i < LEAF; ++i) colv[i] = (i < s && j < s) ? A[(int64_t)(r0 + i) * n + r0 + j] : T(0);
+#pragma unroll
+    for (int i = 0; i < LEAF; ++i) L[i][j] = colv[i];
   }
   __syncthreads();
   const int c = blockIdx.x * 64 + threadIdx.x;
@@ -418,10 +452,13 @@ struct Rec {
     const size_t shm = (size_t)m * sizeof(int);
     dim3 grid((unsigned)count), block(PT);
     ProfScope ps(kProfPanel, (double)count * m * w * w, 2.0 * sizeof(T) * count * m * w, s);
-    if (m <= PT) hipLaunchKernelGGL((leaf_kernel<T, 1>), grid, block, shm, s, la);
-    else if (m <= 2 * PT) hipLaunchKernelGGL((leaf_kernel<T, 2>), grid, block, shm, s, la);
-    else if (m <= 4 * PT) hipLaunchKernelGGL((leaf_kernel<T, 4>), grid, block, shm, s, la);
-    else hipLaunchKernelGGL((leaf_kernel<T, 8>), grid, block, shm, s, la);
+    // inner block width: as wide as the registers allow (fewer block-level
+    // global round trips per leaf)
+    constexpr bool F = sizeof(T) == 4;
+    if (m <= PT) hipLaunchKernelGGL((leaf_kernel<T, 1, F ? 32 : 16>), grid, block, shm, s, la);
+    else if (m <= 2 * PT) hipLaunchKernelGGL((leaf_kernel<T, 2, F ? 32 : 16>), grid, block, shm, s, la);
+    else if (m <= 4 * PT) hipLaunchKernelGGL((leaf_kernel<T, 4, F ? 16 : 8>), grid, block, shm, s, la);
+    else hipLaunchKernelGGL((leaf_kernel<T, 8, F ? 8 : 4>), grid, block, shm, s, la);
     NPPH_HIP(hipGetLastError());
     return NPPH_OK;
   }
@@ -487,16 +524,36 @@ struct Rec {
   }
 };
 
-}  // namespace
 
-int max_blocked_n() { return 8 * PT; }
+// Sub-batch streams: a stack is factored as up to kGroups sub-batches on
+// separate streams (default priority), each with its own workspace set, so the
+// latency-bound leaves of one group overlap the GEMMs of another.
+constexpr int kGroups = 4;
+struct GroupPool {
+  std::once_flag once;
+  std::mutex mu;
+  int status = NPPH_ERR_HIP;
+  hipStream_t st[kGroups] = {};
+  WsSet* ws[kGroups] = {};
+  hipEvent_t ev[kGroups + 1] = {};
+};
+GroupPool& group_pool() {
+  static GroupPool p;
+  std::call_once(p.once, [] {
+    for (int g = 0; g < kGroups; ++g) {
+      if (hipStreamCreateWithFlags(&p.st[g], hipStreamNonBlocking) != hipSuccess) return;
+      p.ws[g] = ws_create(p.st[g]);
+    }
+    for (int g = 0; g <= kGroups; ++g)
+      if (hipEventCreateWithFlags(&p.ev[g], hipEventDisableTiming) != hipSuccess) return;
+    p.status = NPPH_OK;
+  });
+  return p;
+}
 
 template <class T>
-int blocked_getrf_perm(int64_t count, int n, T* A, int64_t strideA, int32_t* ipiv, int32_t* perm, int32_t* info,
-                       int bf16_terms, const int32_t* mask, hipStream_t s) {
-  if (count <= 0 || n <= 0) return NPPH_OK;
-  if (n > max_blocked_n()) { set_error("LU: n=%d > %d", n, max_blocked_n()); return NPPH_ERR_UNSUPPORTED; }
-  if (count > 65535) { set_error("LU: count > 65535"); return NPPH_ERR_INVALID; }
+int getrf_group(int64_t count, int n, T* A, int64_t strideA, int32_t* ipiv, int32_t* perm, int32_t* info,
+                int bf16_terms, const int32_t* mask, hipStream_t s) {
   Rec<T> r;
   r.count = count;
   r.n = n;
@@ -521,6 +578,138 @@ int blocked_getrf_perm(int64_t count, int n, T* A, int64_t strideA, int32_t* ipi
     NPPH_HIP(hipGetLastError());
   }
   return r.getrf(0, n);
+}
+
+}  // namespace
+
+int max_blocked_n() { return 8 * PT; }
+
+namespace {
+
+// Sub-batches per factorisation (NUMPP_HIP_LU_GROUPS overrides).
+int groups_for(int64_t count, int n) {
+  static const int env = [] {
+    const char* e = std::getenv("NUMPP_HIP_LU_GROUPS");
+    return e ? std::atoi(e) : -1;
+  }();
+  int G = env > 0 ? env : (count >= 16 ? kGroups : (count >= 4 ? 2 : 1));
+  if (n < 256) G = 1;
+  return std::max(1, std::min<int>(G, (int)std::min<int64_t>(count, kGroups)));
+}
+
+// Issues the whole factorisation on s, forked over G sub-batch streams.
+template <class T>
+int issue_factor(int G, int64_t count, int n, T* A, int64_t strideA, int32_t* ipiv, int32_t* perm, int32_t* info,
+                 int bf16_terms, const int32_t* mask, hipStream_t s) {
+  if (G == 1) return getrf_group<T>(count, n, A, strideA, ipiv, perm, info, bf16_terms, mask, s);
+  GroupPool& P = group_pool();
+  if (P.status != NPPH_OK) return getrf_group<T>(count, n, A, strideA, ipiv, perm, info, bf16_terms, mask, s);
+  NPPH_HIP(hipEventRecord(P.ev[kGroups], s));
+  int st = NPPH_OK;
+  for (int g = 0; g < G; ++g) {
+    const int64_t c0 = count * g / G, c1 = count * (g + 1) / G;
+    NPPH_HIP(hipStreamWaitEvent(P.st[g], P.ev[kGroups], 0));
+    ws_bind(P.ws[g]);
+    if (st == NPPH_OK)
+      st = getrf_group<T>(c1 - c0, n, A + c0 * strideA, strideA, ipiv + c0 * n, perm + c0 * n, info + c0, bf16_terms,
+                          mask ? mask + c0 : nullptr, P.st[g]);
+    ws_bind(nullptr);
+    NPPH_HIP(hipEventRecord(P.ev[g], P.st[g]));
+    NPPH_HIP(hipStreamWaitEvent(s, P.ev[g], 0));
+  }
+  return st;
+}
+
+// Cached hipGraphs of whole factorisations, keyed by shape, pointers and the
+// workspace generation. A new key runs eagerly once (sizing every workspace);
+// its second call is captured and every later call replays the graph.
+struct GraphKey {
+  int tsize;
+  int64_t count;
+  int n, bf16, G;
+  const void *A, *ipiv, *perm, *info, *mask;
+  int64_t strideA;
+  uint64_t gen;
+  hipStream_t s;
+  bool operator==(const GraphKey& o) const {
+    return tsize == o.tsize && count == o.count && n == o.n && bf16 == o.bf16 && G == o.G && A == o.A &&
+           ipiv == o.ipiv && perm == o.perm && info == o.info && mask == o.mask && strideA == o.strideA &&
+           gen == o.gen && s == o.s;
+  }
+};
+struct GraphEntry {
+  GraphKey key;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+std::mutex g_graph_mu;
+std::vector<GraphEntry> g_graphs;
+std::vector<GraphKey> g_seen;
+constexpr size_t kMaxGraphs = 8;
+
+bool graphs_enabled() {
+  static const bool on = [] {
+    const char* e = std::getenv("NUMPP_HIP_LU_GRAPHS");
+    return !(e && e[0] == '0');
+  }();
+  return on;
+}
+
+}  // namespace
+
+template <class T>
+int blocked_getrf_perm(int64_t count, int n, T* A, int64_t strideA, int32_t* ipiv, int32_t* perm, int32_t* info,
+                       int bf16_terms, const int32_t* mask, hipStream_t s) {
+  if (count <= 0 || n <= 0) return NPPH_OK;
+  if (n > max_blocked_n()) { set_error("LU: n=%d > %d", n, max_blocked_n()); return NPPH_ERR_UNSUPPORTED; }
+  if (count > 65535) { set_error("LU: count > 65535"); return NPPH_ERR_INVALID; }
+  // A caller with its own bound workspace (a serving worker) or an outer
+  // capture gets a plain eager issue on its stream.
+  if (ws_current() != nullptr || stream_capturing(s))
+    return getrf_group<T>(count, n, A, strideA, ipiv, perm, info, bf16_terms, mask, s);
+  const int G = groups_for(count, n);
+  if (!graphs_enabled()) return issue_factor<T>(G, count, n, A, strideA, ipiv, perm, info, bf16_terms, mask, s);
+  GraphKey key{(int)sizeof(T), count, n, bf16_terms, G, A, ipiv, perm, info, mask, strideA, workspace_generation(), s};
+  std::lock_guard<std::mutex> lk(g_graph_mu);
+  for (auto& e : g_graphs)
+    if (e.key == key) {
+      NPPH_HIP(hipGraphLaunch(e.exec, s));
+      return NPPH_OK;
+    }
+  bool seen = false;
+  for (auto& k : g_seen) seen = seen || (k == key);
+  if (!seen) {
+    const int st = issue_factor<T>(G, count, n, A, strideA, ipiv, perm, info, bf16_terms, mask, s);
+    if (st == NPPH_OK) {
+      key.gen = workspace_generation();  // the eager run may have grown the workspaces
+      if (g_seen.size() >= 4 * kMaxGraphs) g_seen.erase(g_seen.begin());
+      g_seen.push_back(key);
+    }
+    return st;
+  }
+  NPPH_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  const int st = issue_factor<T>(G, count, n, A, strideA, ipiv, perm, info, bf16_terms, mask, s);
+  hipGraph_t graph = nullptr;
+  const hipError_t ec = hipStreamEndCapture(s, &graph);
+  if (st != NPPH_OK || ec != hipSuccess || workspace_generation() != key.gen) {
+    if (graph) (void)hipGraphDestroy(graph);
+    if (st != NPPH_OK) return st;
+    // capture failed: fall back to an eager issue of the same work
+    return issue_factor<T>(G, count, n, A, strideA, ipiv, perm, info, bf16_terms, mask, s);
+  }
+  GraphEntry e;
+  e.key = key;
+  e.graph = graph;
+  NPPH_HIP(hipGraphInstantiate(&e.exec, graph, nullptr, nullptr, 0));
+  if (g_graphs.size() >= kMaxGraphs) {
+    (void)hipStreamSynchronize(s);
+    (void)hipGraphExecDestroy(g_graphs.front().exec);
+    (void)hipGraphDestroy(g_graphs.front().graph);
+    g_graphs.erase(g_graphs.begin());
+  }
+  g_graphs.push_back(e);
+  NPPH_HIP(hipGraphLaunch(e.exec, s));
+  return NPPH_OK;
 }
 
 template int blocked_getrf_perm<double>(int64_t, int, double*, int64_t, int32_t*, int32_t*, int32_t*, int,

@@ -141,82 +141,90 @@ __global__ void count_sweep_kernel(int64_t count, const int32_t* active, int32_t
   if (i < count && active[i] == 1) iters[i] += 1;
 }
 
-// R = B - A X for one 8-column RHS chunk and 32 rows per workgroup (8 per
-// wave). X chunks of 128 rows are staged in LDS and shared by the 4 waves;
-// each lane owns two adjacent A columns per chunk (one 16-byte load per row).
-// The 64 partial sums per lane (8 rows x 8 columns) are combined with a
-// halving butterfly, after which lane L owns (row L/8, column L%8).
-constexpr int RES_ROWS = 32, RES_JC = 512, RES_NC = 8;
-__global__ __launch_bounds__(256) void residual_kernel(int n, int nrhs, const double* Ab, const double* Xb,
-                                                       const double* Bb, double* Rb, double* rmax, double* xmax,
-                                                       const int32_t* active) {
+// R = B - A X in two passes. Pass 1 (residual_partial_kernel): one
+// workgroup per (16-row block, matrix, 512-column chunk x 8-RHS chunk) issues
+// its A loads (4 rows per wave, two adjacent columns per lane in each of four
+// 128-column slices, 16-byte loads) before staging its X chunk in LDS, so the
+// A traffic is in flight while X arrives; its 32 partial sums per lane are
+// combined with an xor step and a halving butterfly and written as one
+// partial per (row, column, chunk). Pass 2 (residual_finish_kernel) adds the
+// partials in chunk order (deterministic), forms r = b - sum and the per-column
+// max |r|. The block of matrix rows with index 0 also reports max |x| per column.
+constexpr int RES_RPW = 4, RES_ROWS = 4 * RES_RPW, RES_JC = 512, RES_NC = 8;
+__global__ __launch_bounds__(256) void residual_partial_kernel(int n, int nrhs, int nchunk, const double* Ab,
+                                                               const double* Xb, double* part, double* xmax,
+                                                               const int32_t* active) {
   const int64_t mat = blockIdx.y;
   if (active && active[mat] != 1) return;
-  const int c0 = blockIdx.z * RES_NC;
+  const int ch = blockIdx.z % nchunk, zc = blockIdx.z / nchunk;
+  const int c0 = zc * RES_NC;
   const int nc = min(RES_NC, nrhs - c0);
+  const int jc = ch * RES_JC;
   const double* __restrict__ A = Ab + mat * (int64_t)n * n;
   const double* __restrict__ X = Xb + mat * (int64_t)n * nrhs;
-  const double* __restrict__ B = Bb + mat * (int64_t)n * nrhs;
-  double* __restrict__ R = Rb + mat * (int64_t)n * nrhs;
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-  const int row0 = blockIdx.x * RES_ROWS + wave * 8;
+  const int row0 = blockIdx.x * RES_ROWS + wave * RES_RPW;
   __shared__ double xs[RES_JC][RES_NC];
-  double acc[8][RES_NC];
-#pragma unroll
-  for (int r = 0; r < 8; ++r)
-#pragma unroll
-    for (int c = 0; c < RES_NC; ++c) acc[r][c] = 0.0;
-  double xm = 0.0;
   const bool even = (n & 1) == 0;
-  for (int jc = 0; jc < n; jc += RES_JC) {
+  double a0[RES_JC / 128][RES_RPW], a1[RES_JC / 128][RES_RPW];
 #pragma unroll
-    for (int q = 0; q < (RES_JC * RES_NC) / 256; ++q) {
-      const int e = t + 256 * q;
-      const int j = e / RES_NC, c = e % RES_NC;
-      const double v = (jc + j < n && c < nc) ? X[(int64_t)(jc + j) * nrhs + c0 + c] : 0.0;
-      xs[j][c] = v;
-      xm = fmax(xm, fabs(v));
-    }
-    __syncthreads();
+  for (int sub = 0; sub < RES_JC / 128; ++sub) {
+    const int j0 = jc + sub * 128 + 2 * lane;
 #pragma unroll
-    for (int sub = 0; sub < RES_JC / 128; ++sub) {
-      const int jl = sub * 128 + 2 * lane;
-      const int j0 = jc + jl;
-      if (j0 >= n) break;
-      double a0[8], a1[8];
-#pragma unroll
-      for (int r = 0; r < 8; ++r) {
-        const int row = row0 + r;
-        a0[r] = 0.0;
-        a1[r] = 0.0;
-        if (row < n) {
-          const double* ar = A + (int64_t)row * n;
-          if (even && j0 + 1 < n) {
-            const double2 v = *reinterpret_cast<const double2*>(ar + j0);
-            a0[r] = v.x; a1[r] = v.y;
-          } else {
-            a0[r] = ar[j0];
-            a1[r] = j0 + 1 < n ? ar[j0 + 1] : 0.0;
-          }
+    for (int r = 0; r < RES_RPW; ++r) {
+      const int row = row0 + r;
+      a0[sub][r] = 0.0;
+      a1[sub][r] = 0.0;
+      if (row < n && j0 < n) {
+        const double* ar = A + (int64_t)row * n;
+        if (even) {
+          const double2 v = *reinterpret_cast<const double2*>(ar + j0);
+          a0[sub][r] = v.x;
+          a1[sub][r] = v.y;
+        } else {
+          a0[sub][r] = ar[j0];
+          a1[sub][r] = j0 + 1 < n ? ar[j0 + 1] : 0.0;
         }
       }
-#pragma unroll
-      for (int c = 0; c < RES_NC; ++c) {
-        const double x0 = xs[jl][c], x1 = xs[jl + 1][c];
-#pragma unroll
-        for (int r = 0; r < 8; ++r) acc[r][c] = fma(a1[r], x1, fma(a0[r], x0, acc[r][c]));
-      }
     }
-    __syncthreads();
   }
-  // Halving butterfly: 64 values over 64 lanes -> lane L holds value L.
-  double v[64];
+  double xm = 0.0;
 #pragma unroll
-  for (int r = 0; r < 8; ++r)
+  for (int q = 0; q < (RES_JC * RES_NC) / 256; ++q) {
+    const int e = t + 256 * q;
+    const int j = e / RES_NC, c = e % RES_NC;
+    const double v = (jc + j < n && c < nc) ? X[(int64_t)(jc + j) * nrhs + c0 + c] : 0.0;
+    xs[j][c] = v;
+    xm = fmax(xm, fabs(v));
+  }
+  if (blockIdx.x == 0) {
+    const int cc = t % RES_NC;
+    if (cc < nc) atomic_max_nonneg(xmax + mat * nrhs + c0 + cc, xm);
+  }
+  __syncthreads();
+  double acc[RES_RPW][RES_NC];
 #pragma unroll
-    for (int c = 0; c < RES_NC; ++c) v[r * RES_NC + c] = acc[r][c];
+  for (int r = 0; r < RES_RPW; ++r)
 #pragma unroll
-  for (int h = 32; h >= 1; h >>= 1) {
+    for (int c = 0; c < RES_NC; ++c) acc[r][c] = 0.0;
+#pragma unroll
+  for (int sub = 0; sub < RES_JC / 128; ++sub) {
+    const int jl = sub * 128 + 2 * lane;
+#pragma unroll
+    for (int c = 0; c < RES_NC; ++c) {
+      const double x0 = xs[jl][c], x1 = xs[jl + 1][c];
+#pragma unroll
+      for (int r = 0; r < RES_RPW; ++r) acc[r][c] = fma(a1[sub][r], x1, fma(a0[sub][r], x0, acc[r][c]));
+    }
+  }
+  // 32 values per lane: fold the two half-waves, then a halving butterfly.
+  double v[RES_RPW * RES_NC];
+#pragma unroll
+  for (int r = 0; r < RES_RPW; ++r)
+#pragma unroll
+    for (int c = 0; c < RES_NC; ++c) v[r * RES_NC + c] = acc[r][c] + __shfl_xor(acc[r][c], 32, 64);
+#pragma unroll
+  for (int h = 16; h >= 1; h >>= 1) {
     const bool up = (lane & h) != 0;
 #pragma unroll
     for (int i = 0; i < h; ++i) {
@@ -225,18 +233,24 @@ __global__ __launch_bounds__(256) void residual_kernel(int n, int nrhs, const do
       v[i] = keep + __shfl_xor(send, h, 64);
     }
   }
-  const int r = lane / RES_NC, c = lane % RES_NC;
+  const int r = (lane & 31) / RES_NC, c = lane % RES_NC;
   const int row = row0 + r;
-  if (row < n && c < nc) {
-    const int64_t idx = (int64_t)row * nrhs + c0 + c;
-    const double res = B[idx] - v[0];
-    R[idx] = res;
-    atomic_max_nonneg(rmax + mat * nrhs + c0 + c, fabs(res));
-  }
-  if (blockIdx.x == 0) {
-    const int cc = t % RES_NC;
-    if (cc < nc) atomic_max_nonneg(xmax + mat * nrhs + c0 + cc, xm);
-  }
+  if (lane < 32 && row < n && c < nc)
+    part[((mat * nchunk + ch) * (int64_t)n + row) * nrhs + c0 + c] = v[0];
+}
+
+__global__ void residual_finish_kernel(int64_t count, int n, int nrhs, int nchunk, const double* part,
+                                       const double* Bb, double* Rb, double* rmax, const int32_t* active) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t per = (int64_t)n * nrhs;
+  if (i >= count * per) return;
+  const int64_t mat = i / per, e = i - mat * per;
+  if (active && active[mat] != 1) return;
+  double s = 0.0;
+  for (int ch = 0; ch < nchunk; ++ch) s += part[(mat * nchunk + ch) * per + e];
+  const double res = Bb[i] - s;
+  Rb[i] = res;
+  atomic_max_nonneg(rmax + mat * nrhs + (int)(e % nrhs), fabs(res));
 }
 
 // Convergence test per system (one thread per system). Records eta per column,
@@ -298,9 +312,14 @@ inline dim3 g1(int64_t n, int b = 256) { return dim3((unsigned)ceil_div(n, b)); 
 // Residual + per-(system, column) maxima of |r| and |x| for the current X.
 int residual_launch(int64_t count, int n, int nrhs, const double* A, const double* X, const double* B, double* R,
                     double* rmax, double* xmax, const int32_t* active, hipStream_t s) {
-  dim3 grid((unsigned)ceil_div(n, RES_ROWS), (unsigned)count, (unsigned)ceil_div(nrhs, RES_NC));
+  const int nchunk = (int)ceil_div(n, RES_JC), nz = (int)ceil_div(nrhs, RES_NC);
+  double* part;
+  NPPH_TRY(workspace(kWsPart, sizeof(double) * (size_t)count * nchunk * n * nrhs, (void**)&part));
+  dim3 grid((unsigned)ceil_div(n, RES_ROWS), (unsigned)count, (unsigned)(nchunk * nz));
   ProfScope ps(kProfResidual, 2.0 * count * n * n * nrhs, 8.0 * count * ((double)n * n + 3.0 * n * nrhs), s);
-  hipLaunchKernelGGL(residual_kernel, grid, dim3(256), 0, s, n, nrhs, A, X, B, R, rmax, xmax, active);
+  hipLaunchKernelGGL(residual_partial_kernel, grid, dim3(256), 0, s, n, nrhs, nchunk, A, X, part, xmax, active);
+  hipLaunchKernelGGL(residual_finish_kernel, g1(count * n * nrhs), dim3(256), 0, s, count, n, nrhs, nchunk, part, B,
+                     R, rmax, active);
   NPPH_HIP(hipGetLastError());
   return NPPH_OK;
 }

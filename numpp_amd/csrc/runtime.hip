@@ -100,6 +100,7 @@ void ws_destroy(WsSet* w) {
 }
 
 void ws_bind(WsSet* w) { t_ws = w; }
+WsSet* ws_current() { return t_ws; }
 uint64_t workspace_generation() { return g_ws_gen.load(); }
 
 bool stream_capturing(hipStream_t s) {
