@@ -94,14 +94,41 @@ __device__ __forceinline__ bool piv_better(double v1, int p1, double v0, int p0)
   return v1 > v0 || (v1 == v0 && p1 < p0);
 }
 
-// Wave-wide argmax of (v, pos); every lane receives the winner.
+// One exchange step of the argmax: each lane compares with the lane that the
+// DPP pattern CTRL names (all patterns used stay inside a 16-lane row).
+template <int CTRL>
+__device__ __forceinline__ void argmax_dpp(double& v, int& pos) {
+  const int2 u = __builtin_bit_cast(int2, v);
+  int2 o;
+  o.x = __builtin_amdgcn_mov_dpp(u.x, CTRL, 0xF, 0xF, false);
+  o.y = __builtin_amdgcn_mov_dpp(u.y, CTRL, 0xF, 0xF, false);
+  const int op = __builtin_amdgcn_mov_dpp(pos, CTRL, 0xF, 0xF, false);
+  const double ov = __builtin_bit_cast(double, o);
+  if (piv_better(ov, op, v, pos)) { v = ov; pos = op; }
+}
+
+// Wave-wide argmax of (v, pos); every lane receives the winner. Within each
+// 16-lane row: DPP swaps with the neighbouring lane, the neighbouring pair,
+// the mirrored half-row and the mirrored row (quad_perm [1,0,3,2], quad_perm
+// [2,3,0,1], row_half_mirror, row_mirror) leave the row's winner in all of
+// its lanes; the four row winners are then read with v_readlane. The order is
+// total (largest value, then smallest position), so the result does not
+// depend on the reduction tree.
 __device__ __forceinline__ void wave_argmax(double& v, int& pos) {
+  argmax_dpp<0xB1>(v, pos);
+  argmax_dpp<0x4E>(v, pos);
+  argmax_dpp<0x141>(v, pos);
+  argmax_dpp<0x140>(v, pos);
+  double bv = readlane_d(v, 0);
+  int bp = __builtin_amdgcn_readlane(pos, 0);
 #pragma unroll
-  for (int m = 1; m < kWave; m <<= 1) {
-    double ov = shfl_xor_d(v, m);
-    int op = shfl_xor_i(pos, m);
-    if (piv_better(ov, op, v, pos)) { v = ov; pos = op; }
+  for (int r = 1; r < 4; ++r) {
+    const double ov = readlane_d(v, 16 * r);
+    const int op = __builtin_amdgcn_readlane(pos, 16 * r);
+    if (piv_better(ov, op, bv, bp)) { bv = ov; bp = op; }
   }
+  v = bv;
+  pos = bp;
 }
 
 }  // namespace npph

@@ -30,6 +30,7 @@
 #include <algorithm>
 #include <cstdlib>
 #include <mutex>
+#include <utility>
 #include <vector>
 
 #include "common.hpp"
@@ -60,7 +61,7 @@ struct LeafArgs {
   const int32_t* mask;
 };
 
-template <class T, int R, int IB>
+template <class T, int R, int IB, bool FULL>
 __global__ __launch_bounds__(PT) void leaf_kernel(LeafArgs<T> a) {
   const int64_t mat = blockIdx.x;
   if (a.mask && a.mask[mat] == 0) return;
@@ -71,6 +72,7 @@ __global__ __launch_bounds__(PT) void leaf_kernel(LeafArgs<T> a) {
   int32_t* perm = a.perm + mat * n;
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   constexpr int QC = (R >= 4 && sizeof(T) == 8) ? 4 : 8;  // L columns per update chunk
+  static_assert(IB % QC == 0 && LEAF % IB == 0, "inner blocks must hold whole update chunks");
 
   __shared__ T L11[LEAF][LEAF + 1];
   __shared__ T Ublk[LEAF][IB];
@@ -89,7 +91,7 @@ __global__ __launch_bounds__(PT) void leaf_kernel(LeafArgs<T> a) {
   __syncthreads();
 
   for (int jb = 0; jb < w; jb += IB) {
-    const int ib = min(IB, w - jb);
+    const int ib = FULL ? IB : min(IB, w - jb);  // FULL: w is a multiple of IB
     const int cb = k0 + jb;  // first global column of the block
     T v[R][IB];
     int pos[R];
@@ -100,11 +102,25 @@ __global__ __launch_bounds__(PT) void leaf_kernel(LeafArgs<T> a) {
       pos[s] = p0 < m ? p0 : (1 << 28);
       const T* src = A + (int64_t)(k0 + (p0 < m ? pid[p0] : 0)) * n + cb;
 #pragma unroll
-      for (int c = 0; c < IB; ++c) v[s][c] = (p0 < m && c < ib) ? src[c] : T(0);
+      // the row index is clamped, so the loads need no row guard; padding
+      // slots (pos >= m) are excluded from the pivot search and every write
+      for (int c = 0; c < IB; ++c) v[s][c] = (FULL || c < ib) ? src[c] : T(0);
     }
-    for (int e = t; e < jb * IB; e += PT) {
-      const int q = e / IB, c = e - q * IB;
-      Ublk[q][c] = c < ib ? A[(int64_t)(k0 + pid[q]) * n + cb + c] : T(0);
+    {
+      // U rows of the block (loads of every pass issued before any use)
+      constexpr int UIT = ((LEAF - IB) * IB + PT - 1) / PT;
+      T u[UIT];
+#pragma unroll
+      for (int i = 0; i < UIT; ++i) {
+        const int e = t + i * PT, q = e / IB, c = e % IB;
+        const bool in = e < jb * IB && c < ib;
+        u[i] = in ? A[(int64_t)(k0 + pid[in ? q : 0]) * n + cb + c] : T(0);
+      }
+#pragma unroll
+      for (int i = 0; i < UIT; ++i) {
+        const int e = t + i * PT;
+        if (e < jb * IB) Ublk[e / IB][e % IB] = u[i];
+      }
     }
     __syncthreads();
     // ---- U rows [0, jb) of the block: wave w solves columns w, w+8, ...
@@ -132,7 +148,10 @@ __global__ __launch_bounds__(PT) void leaf_kernel(LeafArgs<T> a) {
         for (int s = 0; s < R; ++s) {
           const T* lr = A + (int64_t)(k0 + (pos[s] < m ? pos[s] : 0)) * n + k0 + q0;
 #pragma unroll
-          for (int q = 0; q < QC; ++q) dst[s][q] = (pos[s] < m && q0 + q < jb) ? lr[q] : T(0);
+          // jb is a multiple of IB and of QC, so a chunk is all inside; rows
+          // past m (clamped to row 0) only feed padding slots, which are
+          // never read back
+          for (int q = 0; q < QC; ++q) dst[s][q] = lr[q];
         }
       };
       auto applyc = [&](const T (&src)[R][QC], int q0) {
@@ -156,14 +175,14 @@ __global__ __launch_bounds__(PT) void leaf_kernel(LeafArgs<T> a) {
         if (q0 + QC < jb) applyc(lb, q0 + QC);
       }
     }
-    // ---- factor the block's columns, one barrier per column. The column
-    // index is a runtime value; register rows are read and written through
-    // select chains over compile-time indices so they stay in registers.
+    // ---- factor the block's columns, one barrier per column
     int opos[R];
 #pragma unroll
     for (int s = 0; s < R; ++s) opos[s] = pos[s];
-#pragma unroll 1
-    for (int c = 0; c < ib; ++c) {
+    // (compile-time column index: one copy of the step per column, so the
+    // register rows are addressed statically)
+    auto fstep = [&]<int c>() __attribute__((always_inline)) {
+      if (c >= ib) return;
       const int col = jb + c;  // current leaf-relative step
       const int buf = c & 1;
       double bv = -1.0;
@@ -171,10 +190,7 @@ __global__ __launch_bounds__(PT) void leaf_kernel(LeafArgs<T> a) {
 #pragma unroll
       for (int s = 0; s < R; ++s) {
         if (pos[s] >= col && pos[s] < m) {
-          T vc = v[s][0];
-#pragma unroll
-          for (int j = 1; j < IB; ++j) vc = (j == c) ? v[s][j] : vc;
-          const double av = fabs((double)vc);
+          const double av = fabs((double)v[s][c]);
           if (piv_better(av, pos[s], bv, bp)) { bv = av; bp = pos[s]; bs = s; }
         }
       }
@@ -189,7 +205,7 @@ __global__ __launch_bounds__(PT) void leaf_kernel(LeafArgs<T> a) {
         for (int s = 0; s < R; ++s)
           if (s == bs) {
 #pragma unroll
-            for (int j = 0; j < IB; ++j) red_row[buf][wave][j] = v[s][j];
+            for (int j = c; j < IB; ++j) red_row[buf][wave][j] = v[s][j];
           }
       } else if (lane == 0 && wv < 0.0) {
         red_v[buf][wave] = -1.0;
@@ -208,12 +224,7 @@ __global__ __launch_bounds__(PT) void leaf_kernel(LeafArgs<T> a) {
       const int p = bp;
       if (bv == 0.0 && t == 0 && sing_s == 0) sing_s = k0 + col + 1;
       if (t == 0) ipiv[k0 + col] = k0 + p;
-      T prow[IB];
-#pragma unroll
-      for (int j = 0; j < IB; ++j) prow[j] = red_row[buf][wbest][j];
-      T piv = prow[0];
-#pragma unroll
-      for (int j = 1; j < IB; ++j) piv = (j == c) ? prow[j] : piv;
+      const T piv = red_row[buf][wbest][c];
 #pragma unroll
       for (int s = 0; s < R; ++s) {
         if (pos[s] == p) pos[s] = col;
@@ -222,18 +233,16 @@ __global__ __launch_bounds__(PT) void leaf_kernel(LeafArgs<T> a) {
 #pragma unroll
       for (int s = 0; s < R; ++s) {
         if (pos[s] > col && pos[s] < m) {
-          T vc = v[s][0];
+          const T l = piv != T(0) ? v[s][c] / piv : T(0);
+          v[s][c] = l;
 #pragma unroll
-          for (int j = 1; j < IB; ++j) vc = (j == c) ? v[s][j] : vc;
-          const T l = piv != T(0) ? vc / piv : T(0);
-#pragma unroll
-          for (int j = 0; j < IB; ++j) {
-            const T upd = v[s][j] - l * prow[j];
-            v[s][j] = (j == c) ? l : ((j > c) ? upd : v[s][j]);
-          }
+          for (int j = c + 1; j < IB; ++j) v[s][j] = v[s][j] - l * red_row[buf][wbest][j];
         }
       }
-    }
+    };
+    [&]<int... C>(std::integer_sequence<int, C...>) __attribute__((always_inline)) {
+      (fstep.template operator()<C>(), ...);
+    }(std::make_integer_sequence<int, IB>{});
     // ---- L11 rows of the block's new pivot rows: block columns from
     // registers, left columns (from their pre-block places) loaded by all
     // threads in parallel
@@ -248,9 +257,19 @@ __global__ __launch_bounds__(PT) void leaf_kernel(LeafArgs<T> a) {
       }
     }
     __syncthreads();
-    for (int e = t; e < ib * jb; e += PT) {
-      const int rr = e / jb, q = e - rr * jb;
-      L11[jb + rr][q] = A[(int64_t)(k0 + stage_pid[rr]) * n + k0 + q];
+    if (jb > 0) {
+      constexpr int LIT = (IB * (LEAF - IB) + PT - 1) / PT;
+      T l[LIT];
+#pragma unroll
+      for (int i = 0; i < LIT; ++i) {
+        const int e = t + i * PT, rr = e / jb, q = e - rr * jb;
+        l[i] = e < ib * jb ? A[(int64_t)(k0 + stage_pid[rr]) * n + k0 + q] : T(0);
+      }
+#pragma unroll
+      for (int i = 0; i < LIT; ++i) {
+        const int e = t + i * PT, rr = e / jb, q = e - rr * jb;
+        if (e < ib * jb) L11[jb + rr][q] = l[i];
+      }
     }
     __syncthreads();
     // ---- write the block at final places; list rows whose place changed
@@ -272,20 +291,38 @@ __global__ __launch_bounds__(PT) void leaf_kernel(LeafArgs<T> a) {
     // ---- move the up-to-date left columns, perm and pid of those rows
     const int cnt = mv_cnt;
     const int oc = jb;  // left columns [k0, cb)
-    for (int e = t; e < cnt * (oc + 2); e += PT) {
-      const int q = e / (oc + 2), j = e - q * (oc + 2);
-      const int src = mv_old[q];
-      if (j < oc) stage[q][j] = A[(int64_t)(k0 + src) * n + k0 + j];
-      else if (j == oc) stage_perm[q] = perm[k0 + src];
-      else stage_pid[q] = pid[src];
-    }
-    __syncthreads();
-    for (int e = t; e < cnt * (oc + 2); e += PT) {
-      const int q = e / (oc + 2), j = e - q * (oc + 2);
-      const int dst = mv_new[q];
-      if (j < oc) A[(int64_t)(k0 + dst) * n + k0 + j] = stage[q][j];
-      else if (j == oc) perm[k0 + dst] = stage_perm[q];
-      else pid[dst] = stage_pid[q];
+    constexpr int MIT = (2 * IB * (LEAF - IB + 2) + PT - 1) / PT;
+    const int tot = cnt * (oc + 2);
+    if (tot > 0) {
+      T mvv[MIT];
+      int mvi[MIT];
+#pragma unroll
+      for (int i = 0; i < MIT; ++i) {
+        const int e = t + i * PT, q = e / (oc + 2), j = e - q * (oc + 2);
+        const int src = e < tot ? mv_old[q] : 0;
+        mvv[i] = (e < tot && j < oc) ? A[(int64_t)(k0 + src) * n + k0 + j] : T(0);
+        mvi[i] = e < tot ? (j == oc ? perm[k0 + src] : (j > oc ? pid[src] : 0)) : 0;
+      }
+#pragma unroll
+      for (int i = 0; i < MIT; ++i) {
+        const int e = t + i * PT, q = e / (oc + 2), j = e - q * (oc + 2);
+        if (e < tot) {
+          if (j < oc) stage[q][j] = mvv[i];
+          else if (j == oc) stage_perm[q] = mvi[i];
+          else stage_pid[q] = mvi[i];
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int i = 0; i < MIT; ++i) {
+        const int e = t + i * PT, q = e / (oc + 2), j = e - q * (oc + 2);
+        if (e < tot) {
+          const int dst = mv_new[q];
+          if (j < oc) A[(int64_t)(k0 + dst) * n + k0 + j] = stage[q][j];
+          else if (j == oc) perm[k0 + dst] = stage_perm[q];
+          else pid[dst] = stage_pid[q];
+        }
+      }
     }
     __syncthreads();
   }
@@ -445,6 +482,14 @@ struct Rec {
     return h >= w ? w - LEAF : h;
   }
 
+  // FULL instantiation when every inner block of the panel is IB wide (the
+  // column guards then fold away and the block's stores are unconditional).
+  template <class U, int R, int IB>
+  void launch_leaf(dim3 grid, dim3 block, size_t shm, const LeafArgs<U>& la) {
+    if (la.w % IB == 0) hipLaunchKernelGGL((leaf_kernel<U, R, IB, true>), grid, block, shm, s, la);
+    else hipLaunchKernelGGL((leaf_kernel<U, R, IB, false>), grid, block, shm, s, la);
+  }
+
   int leaf(int k0, int w) {
     LeafArgs<T> la{A, strideA, n, k0, w, ipiv, perm, info, leafmv + (int64_t)(k0 / LEAF) * LSTRIDE,
                    (int64_t)nleaf * LSTRIDE, mask};
@@ -455,10 +500,10 @@ struct Rec {
     // inner block width: as wide as the registers allow (fewer block-level
     // global round trips per leaf)
     constexpr bool F = sizeof(T) == 4;
-    if (m <= PT) hipLaunchKernelGGL((leaf_kernel<T, 1, F ? 32 : 16>), grid, block, shm, s, la);
-    else if (m <= 2 * PT) hipLaunchKernelGGL((leaf_kernel<T, 2, F ? 32 : 16>), grid, block, shm, s, la);
-    else if (m <= 4 * PT) hipLaunchKernelGGL((leaf_kernel<T, 4, F ? 16 : 8>), grid, block, shm, s, la);
-    else hipLaunchKernelGGL((leaf_kernel<T, 8, F ? 8 : 4>), grid, block, shm, s, la);
+    if (m <= PT) launch_leaf<T, 1, F ? 32 : 16>(grid, block, shm, la);
+    else if (m <= 2 * PT) launch_leaf<T, 2, F ? 32 : 16>(grid, block, shm, la);
+    else if (m <= 4 * PT) launch_leaf<T, 4, F ? 16 : 8>(grid, block, shm, la);
+    else launch_leaf<T, 8, F ? 8 : 4>(grid, block, shm, la);
     NPPH_HIP(hipGetLastError());
     return NPPH_OK;
   }

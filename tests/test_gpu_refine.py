@@ -45,6 +45,16 @@ def test_refined_converges(hip, precision, n, nrhs):
     check_results(a, b, x, berr)
 
 
+@pytest.mark.parametrize("precision", ["fp32", "bf16"])
+def test_refined_converges_tall_panels(hip, precision):
+    # n = 1100: the first low-precision panels span more than 1024 rows
+    a, b = make_stack(1100, 2, 1100, 2, 1e4)
+    x, berr, iters, info = nla.solve_refined(a, b, precision=precision)
+    assert np.all(info == 0)
+    assert np.all(iters >= 1), iters
+    check_results(a, b, x, berr)
+
+
 def test_fp64_precision_reports(hip):
     a, b = make_stack(7, 2, 128, 2, 1e4)
     x, berr, iters, info = nla.solve_refined(a, b, precision="fp64")

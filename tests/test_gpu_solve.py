@@ -114,6 +114,21 @@ def test_large_blocked_2048(hip):
     assert np.abs(x[0] - xo).max() <= 1e-13 * np.linalg.cond(a[0], 1) * np.abs(xo).max()
 
 
+def test_large_blocked_pivots_2100(hip):
+    # n = 2100: the first panels span more than 2048 rows and the last one is
+    # 52 columns wide (partial inner blocks); pivots must equal the oracle's.
+    rng = np.random.default_rng(22)
+    n = 2100
+    a = rng.uniform(-1, 1, (1, n, n))
+    b = rng.standard_normal((1, n, 1))
+    lu, x, ipiv, info = gesv_host(a, b)
+    assert info[0] == 0
+    xo, luo, steps = oracle.solve(a[0], b[0], with_factors=True)
+    np.testing.assert_array_equal(ipiv[0], steps)
+    assert np.abs(lu[0] - luo).max() <= 1e-10 * np.abs(luo).max()
+    assert np.abs(x[0] - xo).max() <= 1e-13 * np.linalg.cond(a[0], 1) * np.abs(xo).max()
+
+
 def test_singular_exact_zero(hip):
     n = 70
     rng = np.random.default_rng(4)
