@@ -713,7 +713,10 @@ int blocked_getrf_perm(int64_t count, int n, T* A, int64_t strideA, int32_t* ipi
   if (ws_current() != nullptr || stream_capturing(s))
     return getrf_group<T>(count, n, A, strideA, ipiv, perm, info, bf16_terms, mask, s);
   const int G = groups_for(count, n);
-  if (!graphs_enabled()) return issue_factor<T>(G, count, n, A, strideA, ipiv, perm, info, bf16_terms, mask, s);
+  // Per-launch profiling needs an eager issue (a replayed graph has no
+  // per-kernel events).
+  if (!graphs_enabled() || prof_enabled())
+    return issue_factor<T>(G, count, n, A, strideA, ipiv, perm, info, bf16_terms, mask, s);
   GraphKey key{(int)sizeof(T), count, n, bf16_terms, G, A, ipiv, perm, info, mask, strideA, workspace_generation(), s};
   std::lock_guard<std::mutex> lk(g_graph_mu);
   for (auto& e : g_graphs)
