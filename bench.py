@@ -14,7 +14,10 @@ in plain FP64 (LU + getrs + backward-error report) is timed beside it.
 `value` = systems solved per second by the faster mixed-precision mode.
 
 Other workloads (--workload): gemm (8192^3 FP64 DGEMM, TFLOP/s), plain64
-(256 x 64 x 64 FP64 stacked solve, solves/s).
+(256 x 64 x 64 FP64 stacked solve, solves/s), uneven (20,000 systems of
+n in [8, 256], FP64 and FP32 + refinement), serve (Poisson-arrival serving
+loop, p50/p99 latency and solves/s per rate), copy (device copy, the
+calibration workload for the HBM byte counters).
 
 Prints ONE JSON line. --gpus above 1 exits: this build runs on one GPU.
 """
@@ -393,13 +396,229 @@ def bench_plain64(args):
             "cpu_baseline": {"value": 256 / dt, "unit": "solves/s", "cores": 1, "kind": "port",
                              "sample": f"full 256-system batch, oracle, 1 of {ncpu} cores ({model})"}}
 
+def bench_uneven(args):
+    """Uneven batch: --systems independent float64 systems (default 20,000),
+    n ~ integers(8, 257), nrhs ~ integers(1, 17) (both drawn as vectors first),
+    then per system in order A = uniform(-1, 1, (n, n)) + n I and
+    B = standard_normal((n, nrhs)); PCG64 seed 4. One planned call per step
+    (size buckets, graph replay); FP64 and FP32 + FP64 refinement."""
+    import ctypes
+
+    import numpp_amd
+    import oracle
+    from numpp_amd.device import DeviceArray, Timer
+    from numpp_amd.vbatch import VPlan
+
+    lib = numpp_amd.require_device()
+    S = args.systems
+    rng = np.random.default_rng(4)
+    t0 = time.time()
+    ns = rng.integers(8, 257, size=S).astype(np.int32)
+    nr = rng.integers(1, 17, size=S).astype(np.int32)
+    offA = np.concatenate([[0], np.cumsum(ns.astype(np.int64) ** 2)])
+    offB = np.concatenate([[0], np.cumsum(ns.astype(np.int64) * nr)])
+    A = np.empty(int(offA[-1]))
+    B = np.empty(int(offB[-1]))
+    for i in range(S):
+        n = int(ns[i])
+        a = rng.uniform(-1.0, 1.0, (n, n))
+        a[np.diag_indices(n)] += n
+        A[offA[i]:offA[i + 1]] = a.ravel()
+        B[offB[i]:offB[i + 1]] = rng.standard_normal((n, int(nr[i]))).ravel()
+    log(f"inputs: {S} systems, {A.nbytes / 1e9:.2f} GB of A, in {time.time() - t0:.1f}s")
+    dA, dB = DeviceArray.from_host(A), DeviceArray.from_host(B)
+    dX = DeviceArray(B.shape)
+    dE = DeviceArray((int(nr.sum()),))
+    dI = DeviceArray((S,), np.int32)
+    dF = DeviceArray((S,), np.int32)
+    results = {}
+    plans = {}
+    for key in [m for m in ("fp64", "fp32") if m in args.modes.split(",")]:
+        t1 = time.time()
+        plan = VPlan(ns, nr, precision=key, tol=1e-12, max_iter=10, fallback=True, autotune=True)
+        plans[key] = plan
+        log(f"{key}: plan with {len(plan.buckets())} buckets in {time.time() - t1:.1f}s")
+        for _ in range(args.warmup):
+            plan.execute(dA, dB, dX, dE, dI, dF, use_graph=True)
+        lib.npph_device_sync()
+        lat = []
+        tm = Timer()
+        tm.start()
+        for _ in range(args.steps):
+            t2 = time.perf_counter()
+            plan.execute(dA, dB, dX, dE, dI, dF, use_graph=True)
+            lib.npph_device_sync()
+            lat.append((time.perf_counter() - t2) * 1e3)
+        ms = tm.stop() / args.steps
+        it = dI.to_host()
+        results[key] = {"ms": ms, "solves_per_s": S / (ms / 1e3), "berr_max": float(dE.to_host().max()),
+                        "info_nonzero": int((dF.to_host() != 0).sum()), "iters_max": int(it.max()),
+                        "iters_min": int(it.min()), "p50_ms": float(np.percentile(lat, 50)),
+                        "p99_ms": float(np.percentile(lat, 99)), "x": dX.to_host()}
+        log(f"{key}: {ms:.3f} ms/step, {results[key]['solves_per_s']:.0f} solves/s, "
+            f"berr_max {results[key]['berr_max']:.2e}, iters {it.min()}..{it.max()}")
+    if "fp64" in results:
+        x64 = results["fp64"]["x"]
+        den = np.maximum.reduceat(np.abs(x64), offB[:-1])
+        for key, r in results.items():
+            num = np.maximum.reduceat(np.abs(r["x"] - x64), offB[:-1])
+            r["fwd_err_vs_fp64_max"] = float((num / den).max())
+    mixed = [k for k in results if k != "fp64" and results[k]["berr_max"] <= 1e-12]
+    best = min(mixed, key=lambda k: results[k]["ms"]) if mixed else min(results, key=lambda k: results[k]["ms"])
+    lib.npph_prof_reset()
+    lib.npph_prof_enable(1)
+    plans[best].execute(dA, dB, dX, dE, dI, dF, use_graph=False)
+    lib.npph_device_sync()
+    lib.npph_prof_enable(0)
+    prof = prof_report(lib, ctypes)
+    lib.npph_prof_reset()
+    dom = max(prof, key=lambda k: prof[k]["ms"])
+    roof = roofline_from(prof, dom, traffic_for(dom, "uneven-" + best))
+    roof["share_of_step"] = prof[dom]["ms"] / results[best]["ms"]
+    for p in plans.values():
+        p.close()
+    cpu = None
+    if not args.no_cpu:
+        idx = np.arange(0, S, max(1, S // 2000))
+        cpus = pin_one_core()
+        t1 = time.perf_counter()
+        for i in idx:
+            n, r = int(ns[i]), int(nr[i])
+            oracle.solve(A[offA[i]:offA[i + 1]].reshape(n, n), B[offB[i]:offB[i + 1]].reshape(n, r))
+        dt = time.perf_counter() - t1
+        unpin(cpus)
+        model, ncpu = cpu_info()
+        cpu = {"value": len(idx) / dt, "unit": "solves/s", "cores": 1, "kind": "port",
+               "sample": f"every {S // len(idx)}th system ({len(idx)} of {S}), FP64 LU+solve, oracle C "
+                         f"restatement -O3 -ffp-contract=off, 1 of {ncpu} cores ({model})", "seconds": dt}
+        log(f"cpu baseline: {dt:.2f}s for {len(idx)} systems")
+    r = results[best]
+    return {"metric": "solves/s + p99 latency, mixed-prec refined vs FP64 LU at backward err <=1e-12",
+            "value": r["solves_per_s"], "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": r["ms"], "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f64" if best == "fp64" else "f32+f64", "data": "synthetic",
+            "config": {"workload": f"uneven batch: {S} float64 systems, n ~ U{{8..256}}, nrhs ~ U{{1..16}}, "
+                                   f"A = U[-1,1) + n I, seed 4", "systems": S, "mode": best, "tol": 1e-12,
+                       "parallelism": "single-gpu"},
+            "p50_ms": r["p50_ms"], "p99_ms": r["p99_ms"],
+            "fp64_solves_per_s": results["fp64"]["solves_per_s"] if "fp64" in results else None,
+            "speedup_vs_fp64": results["fp64"]["ms"] / r["ms"] if "fp64" in results else None,
+            "modes": {k: {kk: v for kk, v in rr.items() if kk != "x"} for k, rr in results.items()},
+            "roofline": roof, "kernel_ms_per_step": {k: round(v["ms"], 4) for k, v in prof.items()},
+            "cpu_baseline": cpu}
+
+
+def bench_serve(args):
+    """Serving loop under Poisson arrivals at each rate in --rates (requests/s).
+    --requests requests (default 160), PCG64 seed 5: n = exp(U(ln 16, ln 4096)),
+    nrhs = round(exp(U(0, ln 64))), kappa = exp(U(ln 1e2, ln 1e8)); A = U diag(s)
+    V^T with Haar U, V and s = logspace(0, -log10 kappa, n). With probability
+    0.3 a request (after the first) reuses the factors kept by a uniformly
+    chosen earlier request with new right-hand sides; every other request keeps
+    its factors. Budget 32e9 bytes (32 GB = 29.8 GiB), 4 workers, batches of up
+    to 8 same-order requests. Mixed (FP32 + refinement) and plain FP64."""
+    import oracle
+    from numpp_amd.serve import Server
+
+    numpp_amd_lib = __import__("numpp_amd").require_device()
+    del numpp_amd_lib
+    R = args.requests
+    rng = np.random.default_rng(5)
+    t0 = time.time()
+    ns = np.exp(rng.uniform(np.log(16), np.log(4096), R)).astype(int)
+    nrs = np.maximum(1, np.rint(np.exp(rng.uniform(0.0, np.log(64), R)))).astype(int)
+    kap = np.exp(rng.uniform(np.log(1e2), np.log(1e8), R))
+    reuse_u = rng.uniform(size=R)
+    owners = []
+    mats, rhss, keys, keeps, owner_of = [], [], [], [], []
+    for i in range(R):
+        if i > 0 and reuse_u[i] < 0.3 and owners:
+            j = owners[int(rng.integers(0, len(owners)))]
+            n = ns[j]
+            mats.append(None)
+            keys.append(j + 1)
+            keeps.append(0)
+            owner_of.append(j)
+        else:
+            n = ns[i]
+            s = np.logspace(0.0, -np.log10(kap[i]), n)
+            a = (haar(rng, n) * s) @ haar(rng, n).T
+            mats.append(a)
+            keys.append(i + 1)
+            keeps.append(1)
+            owner_of.append(i)
+            owners.append(i)
+        rhss.append(rng.standard_normal((n, nrs[i])))
+    nre = sum(m is None for m in mats)
+    log(f"inputs: {R} requests ({nre} reuse), n {ns.min()}..{ns.max()}, in {time.time() - t0:.1f}s")
+    rates = [float(x) for x in args.rates.split(",")]
+    runs = []
+    for prec in [m for m in ("fp32", "fp64") if m in args.modes.split(",")]:
+        # warm-up replay: compiles nothing, but sizes pools and workspaces
+        srv = Server(budget_bytes=32e9, workers=4, max_batch=8, precision=prec)
+        k = min(R, 24)
+        srv.replay(np.zeros(k), mats[:k], rhss[:k], keys[:k], keeps[:k])
+        srv.close()
+        for lam in rates:
+            gaps = np.random.default_rng(50 + int(lam)).exponential(1.0 / lam, R)
+            t_arr = np.cumsum(gaps) - gaps[0]
+            srv = Server(budget_bytes=32e9, workers=4, max_batch=8, precision=prec)
+            out = srv.replay(t_arr, mats, rhss, keys, keeps)
+            st = srv.stats()
+            srv.close()
+            ok = out["req_status"] == 0
+            berr = max(float(np.max(e)) for e in out["berr"])
+            lat = out["latency_ms"][ok]
+            run = {"precision": prec, "rate": lam, "requests": R, "ok": int(ok.sum()), "wall_s": out["wall_s"],
+                   "solves_per_s": int(ok.sum()) / out["wall_s"], "p50_ms": float(np.percentile(lat, 50)),
+                   "p99_ms": float(np.percentile(lat, 99)), "berr_max": berr,
+                   "fallbacks": int((out["iters"] < 0).sum()), "batched": st["batched"],
+                   "peak_gb": st["peak_bytes"] / 1e9}
+            runs.append(run)
+            log(f"{prec} rate {lam:g}/s: {run['solves_per_s']:.1f} solves/s, p50 {run['p50_ms']:.1f} ms, "
+                f"p99 {run['p99_ms']:.1f} ms, berr_max {berr:.2e}, ok {run['ok']}/{R}, "
+                f"fallbacks {run['fallbacks']}, batched {run['batched']}")
+    cpu = None
+    if not args.no_cpu:
+        small = [i for i in range(R) if mats[i] is not None and ns[i] <= 512][:24]
+        cpus = pin_one_core()
+        svc = []
+        for i in small:
+            t1 = time.perf_counter()
+            oracle.solve(mats[i], rhss[i])
+            svc.append((time.perf_counter() - t1) * 1e3)
+        unpin(cpus)
+        model, ncpu = cpu_info()
+        cpu = {"value": len(svc) / (sum(svc) / 1e3), "unit": "solves/s", "cores": 1, "kind": "port",
+               "sample": f"{len(svc)} requests of the trace with n <= 512 served one at a time (FP64 LU+solve, "
+                         f"oracle C -O3 -ffp-contract=off), 1 of {ncpu} cores ({model})",
+               "p50_ms": float(np.percentile(svc, 50)), "p99_ms": float(np.percentile(svc, 99))}
+    top = max(r["rate"] for r in runs)
+    head = [r for r in runs if r["rate"] == top and r["precision"] == "fp32"] or [r for r in runs if r["rate"] == top]
+    h = head[0]
+    plain = [r for r in runs if r["rate"] == top and r["precision"] == "fp64"]
+    return {"metric": "solves/s + p99 latency, mixed-prec refined vs FP64 LU at backward err <=1e-12",
+            "value": h["solves_per_s"], "unit": "solves/s", "n_gpus": 1, "steps": len(runs), "warmup": 1,
+            "ms_per_step": 1e3 * h["wall_s"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f32+f64" if h["precision"] == "fp32" else "f64", "data": "synthetic",
+            "config": {"workload": f"serving: {R} requests, Poisson rates {args.rates}/s, n log-U[16,4096], "
+                                   f"nrhs log-U[1,64], kappa log-U[1e2,1e8], 30% reuse, budget 32e9 B, seed 5",
+                       "mode": h["precision"], "rate": top, "parallelism": "single-gpu"},
+            "p50_ms": h["p50_ms"], "p99_ms": h["p99_ms"],
+            "fp64_solves_per_s": plain[0]["solves_per_s"] if plain else None,
+            "fp64_p99_ms": plain[0]["p99_ms"] if plain else None,
+            "runs": runs, "roofline": None, "cpu_baseline": cpu}
+
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="mixed", choices=["mixed", "gemm", "plain64", "copy"])
+    ap.add_argument("--workload", default="mixed", choices=["mixed", "gemm", "plain64", "copy", "uneven", "serve"])
+    ap.add_argument("--systems", type=int, default=20000, help="uneven workload: number of systems")
+    ap.add_argument("--requests", type=int, default=160, help="serve workload: requests per rate")
+    ap.add_argument("--rates", default="25,100,400", help="serve workload: Poisson rates, requests/s")
     ap.add_argument("--count", type=int, default=64)
     ap.add_argument("--n", type=int, default=2048)
     ap.add_argument("--cpu-sample", type=int, default=16)
@@ -414,7 +633,8 @@ def main():
     if os.environ.get("WORLD_SIZE", "1") not in ("", "1"):
         print("bench.py: multi-rank launch is not supported by this single-GPU build", file=sys.stderr)
         sys.exit(2)
-    fn = {"mixed": bench_mixed, "gemm": bench_gemm, "plain64": bench_plain64, "copy": bench_copy}[args.workload]
+    fn = {"mixed": bench_mixed, "gemm": bench_gemm, "plain64": bench_plain64, "copy": bench_copy,
+          "uneven": bench_uneven, "serve": bench_serve}[args.workload]
     line = fn(args)
     print(json.dumps(line, default=float))
 

@@ -65,6 +65,18 @@ void ws_destroy(WsSet* w);
 void ws_bind(WsSet* w);  // nullptr = the process-wide set
 WsSet* ws_current();     // the calling thread's bound set, or nullptr
 
+// While a PlainIssue is alive on a thread, large factorisations on that thread
+// are issued as plain launches on the caller's stream (no sub-batch streams, no
+// cached graphs): a caller that runs work eagerly and then captures the same
+// work needs both runs to take the same path and touch the same workspaces.
+struct PlainIssue {
+  PlainIssue();
+  ~PlainIssue();
+  PlainIssue(const PlainIssue&) = delete;
+  PlainIssue& operator=(const PlainIssue&) = delete;
+};
+bool plain_issue();
+
 inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
 // ---- wave-level helpers ----

@@ -708,9 +708,9 @@ int blocked_getrf_perm(int64_t count, int n, T* A, int64_t strideA, int32_t* ipi
   if (count <= 0 || n <= 0) return NPPH_OK;
   if (n > max_blocked_n()) { set_error("LU: n=%d > %d", n, max_blocked_n()); return NPPH_ERR_UNSUPPORTED; }
   if (count > 65535) { set_error("LU: count > 65535"); return NPPH_ERR_INVALID; }
-  // A caller with its own bound workspace (a serving worker) or an outer
-  // capture gets a plain eager issue on its stream.
-  if (ws_current() != nullptr || stream_capturing(s))
+  // A caller with its own bound workspace (a serving worker), an outer capture
+  // or a caller that is about to capture gets a plain issue on its stream.
+  if (ws_current() != nullptr || plain_issue() || stream_capturing(s))
     return getrf_group<T>(count, n, A, strideA, ipiv, perm, info, bf16_terms, mask, s);
   const int G = groups_for(count, n);
   // Per-launch profiling needs an eager issue (a replayed graph has no

@@ -99,6 +99,13 @@ void ws_destroy(WsSet* w) {
   delete w;
 }
 
+namespace {
+thread_local int t_plain = 0;
+}
+PlainIssue::PlainIssue() { ++t_plain; }
+PlainIssue::~PlainIssue() { --t_plain; }
+bool plain_issue() { return t_plain > 0; }
+
 void ws_bind(WsSet* w) { t_ws = w; }
 WsSet* ws_current() { return t_ws; }
 uint64_t workspace_generation() { return g_ws_gen.load(); }

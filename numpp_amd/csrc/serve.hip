@@ -16,7 +16,9 @@
 //   column, so a result is bitwise the same alone and in the mix.
 // - Resident factors: a request with keep = 1 leaves (A, factors) on the device
 //   under its key; a later request with the same key skips the factorisation
-//   and only runs the refined solve with new right-hand sides.
+//   and only runs the refined solve with new right-hand sides. A reuse request
+//   whose key is still being factorised by an earlier request (queued or
+//   running) is held back and queued again as soon as the factors are resident.
 // Host buffers passed to submit must stay valid until wait returns.
 #include <algorithm>
 #include <atomic>
@@ -74,6 +76,8 @@ struct Server {
   std::deque<Request*> queue;
   std::unordered_map<int64_t, Request*> reqs;
   std::unordered_map<uint64_t, Resident*> cache;
+  std::unordered_map<uint64_t, int> pending_keep;  // key -> in-flight requests that will keep it
+  std::unordered_map<uint64_t, std::vector<Request*>> deferred;  // reuse requests waiting for their key
   std::list<uint64_t> lru;  // front = most recent
   std::vector<std::thread> workers;
   bool stop = false;
@@ -112,6 +116,28 @@ bool make_room(Server& sv, size_t need, std::vector<Resident*>& freed) {
     freed.push_back(victim);
   }
   return true;
+}
+
+// A request that was to keep factors under `key` has finished (or failed);
+// reuse requests held for that key are queued again once the factors are
+// resident, or failed when no other in-flight request will provide them.
+// Caller holds mu.
+void release_pending(Server& sv, uint64_t key) {
+  auto pk = sv.pending_keep.find(key);
+  if (pk != sv.pending_keep.end() && --pk->second <= 0) sv.pending_keep.erase(pk);
+  auto d = sv.deferred.find(key);
+  if (d == sv.deferred.end()) return;
+  if (sv.cache.count(key)) {
+    for (auto q = d->second.rbegin(); q != d->second.rend(); ++q) sv.queue.push_front(*q);
+    sv.deferred.erase(d);
+  } else if (!sv.pending_keep.count(key)) {
+    for (auto* q : d->second) {
+      q->status = NPPH_ERR_NOT_FOUND;
+      q->done = true;
+      q->t_start = q->t_done = Clock::now();
+    }
+    sv.deferred.erase(d);
+  }
 }
 
 void free_resident(Resident* r, hipStream_t s) {
@@ -238,6 +264,12 @@ void worker_main(Server* sv) {
         auto it = h->key ? sv->cache.find(h->key) : sv->cache.end();
         const bool is_reuse = it != sv->cache.end() && it->second->n == h->n;
         h->bytes = request_bytes(*sv, h->n, h->nrhs, is_reuse);
+        if (!is_reuse && !h->A && sv->pending_keep.count(h->key)) {
+          // its factors are still being computed: hold it until they are resident
+          sv->queue.pop_front();
+          sv->deferred[h->key].push_back(h);
+          continue;
+        }
         if (!is_reuse && !h->A) {
           // the resident factors it referred to were evicted (or never kept)
           sv->queue.pop_front();
@@ -252,6 +284,7 @@ void worker_main(Server* sv) {
           h->status = NPPH_ERR_BUDGET;
           h->done = true;
           h->t_start = h->t_done = Clock::now();
+          if (h->A && h->keep && h->key) release_pending(*sv, h->key);
           sv->cv.notify_all();
           continue;
         }
@@ -269,7 +302,7 @@ void worker_main(Server* sv) {
           // coalesce queued one-shot requests of the same order
           for (auto q = sv->queue.begin(); q != sv->queue.end() && (int)group.size() < sv->cfg.max_batch;) {
             Request* c = *q;
-            if (c->n == h->n && !(c->keep && c->key) && !(c->key && sv->cache.count(c->key))) {
+            if (c->n == h->n && c->A && !(c->keep && c->key) && !(c->key && sv->cache.count(c->key))) {
               const size_t b = request_bytes(*sv, c->n, c->nrhs, false);
               if (sv->used + b <= sv->budget) {
                 c->bytes = b;
@@ -328,6 +361,8 @@ void worker_main(Server* sv) {
           kept = nullptr;
         }
       }
+      for (auto* r : group)
+        if (r->A && r->keep && r->key) release_pending(*sv, r->key);
       sv->cv.notify_all();
     }
     if (kept) free_resident(kept, s);
@@ -380,6 +415,12 @@ int npph_server_destroy(void* srv) {
       r->done = true;
     }
     sv->queue.clear();
+    for (auto& kv : sv->deferred)
+      for (auto* r : kv.second) {
+        r->status = NPPH_ERR_INVALID;
+        r->done = true;
+      }
+    sv->deferred.clear();
     sv->cv.notify_all();
   }
   for (auto& t : sv->workers) t.join();
@@ -408,11 +449,13 @@ int npph_server_submit(void* srv, int64_t n, int64_t nrhs, const double* A, cons
   r->keep = keep;
   r->t_submit = Clock::now();
   std::lock_guard<std::mutex> lk(sv->mu);
-  if (!A && !(key && sv->cache.count(key))) {
+  if (!A && !(key && (sv->cache.count(key) || sv->pending_keep.count(key)))) {
     delete r;
-    set_error("server submit: A is null and key %llu is not resident", (unsigned long long)key);
+    set_error("server submit: A is null and key %llu is neither resident nor being factorised",
+              (unsigned long long)key);
     return NPPH_ERR_NOT_FOUND;
   }
+  if (A && keep && key) sv->pending_keep[key]++;
   r->id = sv->next_id++;
   sv->reqs[r->id] = r;
   sv->queue.push_back(r);

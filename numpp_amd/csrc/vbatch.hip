@@ -411,6 +411,8 @@ int npph_vbatch_execute(void* plan, const double* A, const double* B, double* X,
   hipStream_t s = stream ? (hipStream_t)stream : default_stream();
   const bool graphable = use_graph && p->opts.precision == NPPH_PREC_FP64;
   if (!graphable) return run_plan(p, A, B, X, berr, iters, info, s);
+  // The eager warm-up and the capture issue the same launches on s.
+  PlainIssue plain;
   // First call: run eagerly so every workspace slot reaches its final size.
   if (!p->warmed) {
     NPPH_TRY(run_plan(p, A, B, X, berr, iters, info, s));

@@ -60,6 +60,27 @@ def test_resident_factor_reuse(hip):
     srv.close()
 
 
+def test_reuse_submitted_while_owner_runs(hip):
+    # repeat right-hand sides submitted right behind the request that keeps the
+    # factors: they are held until the factors are resident, then served from them
+    rng = np.random.default_rng(3)
+    n = 700
+    a = cond_matrix(rng, n, 1e4)
+    bs = [rng.standard_normal((n, 2)) for _ in range(4)]
+    srv = Server(budget_bytes=8e9, workers=4, precision="fp32")
+    owner = srv.submit(a, bs[0], key=5, keep=True)
+    reuse = [srv.submit(None, b, key=5) for b in bs[1:]]
+    r0 = srv.wait(owner)
+    rs = [srv.wait(i) for i in reuse]
+    assert r0["status"] == 0 and all(r["status"] == 0 for r in rs)
+    # the same right-hand side through the resident factors after the fact
+    late = srv.wait(srv.submit(None, bs[1], key=5))
+    np.testing.assert_array_equal(late["x"], rs[0]["x"])
+    srv.close()
+    for b, r in zip(bs, [r0] + rs):
+        assert np.all(oracle.backward_error(a, b, r["x"]) <= 1e-12)
+
+
 def test_budget_admission_and_eviction(hip):
     rng = np.random.default_rng(3)
     n = 512

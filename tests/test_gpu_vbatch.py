@@ -84,3 +84,41 @@ def test_alone_equals_in_mix(hip):
     for i in (0, 17, 39):
         x1, _, _, _ = vbatch.solve_uneven([mats[i]], [rhss[i]], "fp64")
         np.testing.assert_array_equal(x1[0], xs[i])
+
+
+FRESH = r'''
+import sys
+import numpy as np
+sys.path.insert(0, sys.argv[1])
+from numpp_amd import vbatch
+from numpp_amd.device import DeviceArray
+rng = np.random.default_rng(8)
+ns = np.concatenate([rng.integers(230, 257, 10), rng.integers(8, 200, 30)])
+mats = [rng.uniform(-1, 1, (n, n)) + n * np.eye(n) for n in ns]
+rhss = [rng.standard_normal((n, 3)) for n in ns]
+A, B, n, nr = vbatch.pack(mats, rhss)
+plan = vbatch.VPlan(n, nr, "fp64")
+dA, dB = DeviceArray.from_host(A), DeviceArray.from_host(B)
+dX, dI = DeviceArray(B.shape), DeviceArray((len(n),), np.int32)
+outs = []
+for rep in range(4):  # eager warm-up, capture, replays
+    dX.copy_from(np.zeros_like(B))
+    plan.execute(dA, dB, dX, dinfo=dI, use_graph=True)
+    outs.append(dX.to_host())
+plan.close()
+for o in outs[1:]:
+    assert np.array_equal(o, outs[0])
+print("fresh-process graph replay ok", flush=True)
+'''
+
+
+def test_graph_capture_fresh_process(hip):
+    # In a new process no workspace has been sized yet: the eager warm-up run
+    # must size exactly what the captured run uses (buckets of order 256 with
+    # enough systems to be factorised over several sub-batch streams otherwise).
+    import os
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, "-c", FRESH, root], capture_output=True, text=True, timeout=180)
+    assert r.returncode == 0, r.stdout + r.stderr
