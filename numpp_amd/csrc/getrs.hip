@@ -4,16 +4,19 @@
 // row permutation, forward substitution with the unit-lower L, back
 // substitution with U dividing by its diagonal.
 //
-// Each sweep is one launch with one 256-thread workgroup per (matrix, 64-row
-// block, 8-column RHS chunk). Block i first accumulates the contributions of
-// the blocks it depends on (L_ij X_j for j < i forward, U_ij X_j for j > i
-// backward) as soon as each X_j is published, reading its 64-row strip of the
-// factor coalesced, then solves its 64 x 64 diagonal block with one wave
-// (lane = row, v_readlane broadcasts; U's diagonal is applied as a multiply by
-// its reciprocal), publishes X_i and raises its flag
-// (release at device scope; readers acquire). Workgroups are dispatched in
-// index order and a block only waits for lower-indexed ones, so every wait
-// ends; a bounded spin reports an error instead of hanging.
+// Each sweep is one launch with one workgroup per (matrix, chunk of CW
+// right-hand-side columns). The workgroup keeps its n x CW slice of the
+// right-hand side in LDS and walks the 64-row blocks in order: it first
+// subtracts the contributions of the blocks already solved (L_ij X_j for
+// j < i forward, U_ij X_j for j > i backward), streaming its 64-row strip of
+// the factor with S threads per row, then solves the 64 x 64 diagonal block
+// with one wave per column (lane = row, v_readlane broadcasts; U's diagonal is
+// applied as a multiply by its reciprocal). Nothing is shared between
+// workgroups, so the sweep needs no inter-workgroup flags (which on this
+// device would need L2 write-back and invalidation at device scope).
+#include <algorithm>
+#include <cstdlib>
+
 #include "common.hpp"
 #include "lu.hpp"
 #include "prof.hpp"
@@ -22,152 +25,152 @@ namespace npph {
 
 namespace {
 
-constexpr int GT = 256;
-constexpr int GB = 64;   // rows per block
-constexpr int GNR = 8;   // RHS columns per chunk
-constexpr unsigned kSpinLimit = 1u << 22;
+constexpr int GB = 64;  // rows per block
 
-// Waits for a published block; gives up (and raises *err) after a bounded
-// number of polls or as soon as another block has given up.
-__device__ __forceinline__ bool wait_flag(const int* f, int* err) {
-  unsigned spins = 0;
-  // Poll relaxed (an acquire per poll would invalidate caches on every spin),
-  // then acquire once the flag is seen.
-  while (__hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) {
-    __builtin_amdgcn_s_sleep(2);
-    if (++spins > kSpinLimit || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
-      atomicExch(err, 1);
-      return false;
-    }
-  }
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-  return true;
-}
-
-// Forward (LOWER = true, unit-lower L, blocks in increasing order, B gathered
-// by perm) or backward (LOWER = false, upper U with division by the diagonal,
-// blocks in decreasing order) sweep. X holds the forward result between sweeps.
-template <class T, bool LOWER>
-__global__ __launch_bounds__(GT) void sweep_kernel(int n, int nrhs, const T* LUb, int64_t strideA,
+template <class T, bool LOWER, int NT, int CW>
+__global__ __launch_bounds__(NT) void sweep_kernel(int n, int nrhs, const T* LUb, int64_t strideA,
                                                    const int32_t* permb, const T* Bb, int64_t strideB, T* Xb,
-                                                   int64_t strideX, int* flags, int* err, const int32_t* info,
-                                                   const int32_t* mask) {
-  const int nblk = (n + GB - 1) / GB;
-  // x = matrix (fastest), y = block: the resident workgroups cover the early
-  // blocks of every matrix, so all chains advance together.
+                                                   int64_t strideX, const int32_t* info, const int32_t* mask) {
+  constexpr int S = NT / GB;       // threads per row in the off-diagonal pass
+  constexpr int NWV = NT / 64;     // waves
+  constexpr int CPW = (CW + NWV - 1) / NWV;  // columns per wave in the diagonal solve
+  constexpr int M = sizeof(T) == 8 ? 16 : 32;  // factor loads in flight per thread
   const int64_t mat = blockIdx.x;
-  const int z = blockIdx.z;
-  int* fl = flags + (mat * gridDim.z + z) * (int64_t)nblk;
   if ((info && info[mat] != 0) || (mask && mask[mat] == 0)) return;
-  const int bi = LOWER ? (int)blockIdx.y : nblk - 1 - (int)blockIdx.y;
-  const int r0 = bi * GB, bs = min(GB, n - r0);
-  const int c0 = z * GNR, nc = min(GNR, nrhs - c0);
+  const int c0 = (int)blockIdx.y * CW, nc = min(CW, nrhs - c0);
   const T* __restrict__ LU = LUb + mat * strideA;
   T* __restrict__ X = Xb + mat * strideX;
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-  const int r = t >> 2, qq = t & 3;  // row in block, quarter of the 64 columns
-  __shared__ T xs[GB][GNR];
-  __shared__ T part[GB][GNR];
-  __shared__ int ok_s;
+  extern __shared__ unsigned char smem_raw[];
+  T* xs = reinterpret_cast<T*>(smem_raw);  // n x CW
+  __shared__ T part[GB][CW];
 
-  T acc[GNR];
+  // right-hand side slice: B gathered by perm (forward) or X (backward); a
+  // thread takes whole rows, 4 rows per pass with their loads in flight
+  {
+    const T* Bm = Bb + mat * strideB;
+    const int32_t* perm = permb + mat * n;
+    for (int i0 = t; i0 < n; i0 += 4 * NT) {
+      int src[4];
 #pragma unroll
-  for (int c = 0; c < GNR; ++c) acc[c] = T(0);
-  if (t == 0) ok_s = 1;
-
-  for (int step = 0; step < nblk; ++step) {
-    const int bj = LOWER ? step : nblk - 1 - step;
-    if (LOWER ? bj >= bi : bj <= bi) break;
-    if (t == 0 && !wait_flag(fl + bj, err)) ok_s = 0;
-    __syncthreads();
-    if (!ok_s) return;
-    const int j0 = bj * GB, js = min(GB, n - j0);
-    for (int e = t; e < GB * GNR; e += GT) {
-      const int q = e / GNR, c = e - q * GNR;
-      xs[q][c] = (q < js && c < nc) ? X[(int64_t)(j0 + q) * nrhs + c0 + c] : T(0);
-    }
-    __syncthreads();
-    if (r < bs) {
-      const T* lr = LU + (int64_t)(r0 + r) * n + j0 + qq * 16;
-#pragma unroll
-      for (int q = 0; q < 16; ++q) {
-        const int qi = qq * 16 + q;
-        const T l = qi < js ? lr[q] : T(0);
-#pragma unroll
-        for (int c = 0; c < GNR; ++c) acc[c] = acc[c] + l * xs[qi][c];
+      for (int u = 0; u < 4; ++u) {
+        const int i = min(i0 + u * NT, n - 1);
+        src[u] = LOWER ? perm[i] : i;
       }
-    }
-    __syncthreads();
-  }
-  // Combine the four quarter sums of each row.
+      T v[4][CW];
 #pragma unroll
-  for (int c = 0; c < GNR; ++c) {
-    acc[c] += __shfl_xor(acc[c], 1, 64);
-    acc[c] += __shfl_xor(acc[c], 2, 64);
-  }
-  if (qq == 0) {
+      for (int u = 0; u < 4; ++u) {
+        const T* br = (LOWER ? Bm : X) + (int64_t)src[u] * nrhs + c0;
 #pragma unroll
-    for (int c = 0; c < GNR; ++c) part[r][c] = acc[c];
-  }
-  __syncthreads();
-  // Diagonal block: wave 0, lane = row.
-  if (wave == 0) {
-    const int i = r0 + lane;
-    T x[GNR];
-    if (LOWER) {
-      const int src = lane < bs ? permb[mat * n + i] : 0;
-      const T* Bm = Bb + mat * strideB;
-#pragma unroll
-      for (int c = 0; c < GNR; ++c) x[c] = (lane < bs && c < nc) ? Bm[(int64_t)src * nrhs + c0 + c] - part[lane][c] : T(0);
-    } else {
-#pragma unroll
-      for (int c = 0; c < GNR; ++c) x[c] = (lane < bs && c < nc) ? X[(int64_t)i * nrhs + c0 + c] - part[lane][c] : T(0);
-    }
-    T row[GB];
-    const T* lr = LU + (int64_t)(lane < bs ? i : r0) * n + r0;
-#pragma unroll
-    for (int q = 0; q < GB; ++q) row[q] = (lane < bs && q < bs) ? lr[q] : T(0);
-    if (LOWER) {
-#pragma unroll
-      for (int q = 0; q < GB; ++q) {
-        const T f = lane > q ? row[q] : T(0);
-#pragma unroll
-        for (int c = 0; c < GNR; ++c) x[c] = x[c] - f * readlane<T>(x[c], q);
+        for (int c = 0; c < CW; ++c) v[u][c] = c < nc ? br[c] : T(0);
       }
-    } else {
-      T d = T(1);
 #pragma unroll
-      for (int q = 0; q < GB; ++q) d = (q == lane && lane < bs) ? row[q] : d;
-      // one division per lane, in parallel; the row's scaling at its step is
-      // then a multiply (within one rounding of lu_solve's division)
-      const T dinv = T(1) / d;
+      for (int u = 0; u < 4; ++u) {
+        const int i = i0 + u * NT;
+        if (i < n) {
 #pragma unroll
-      for (int q = GB - 1; q >= 0; --q) {
-        if (lane == q) {
-#pragma unroll
-          for (int c = 0; c < GNR; ++c) x[c] = x[c] * dinv;
+          for (int c = 0; c < CW; ++c) xs[(int64_t)i * CW + c] = v[u][c];
         }
-        const T f = lane < q ? row[q] : T(0);
-#pragma unroll
-        for (int c = 0; c < GNR; ++c) x[c] = x[c] - f * readlane<T>(x[c], q);
       }
-    }
-    if (lane < bs) {
-#pragma unroll
-      for (int c = 0; c < GNR; ++c)
-        if (c < nc) X[(int64_t)i * nrhs + c0 + c] = x[c];
     }
   }
   __syncthreads();
-  if (t == 0) {
-    __threadfence();
-    __hip_atomic_store(fl + bi, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-  }
-}
 
-__global__ void clear_flags_kernel(int* f, int64_t n) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) f[i] = 0;
+  const int nblk = (n + GB - 1) / GB;
+  const int r = t / S, sg = t - r * S;
+  for (int step = 0; step < nblk; ++step) {
+    const int bi = LOWER ? step : nblk - 1 - step;
+    const int r0 = bi * GB, bs = min(GB, n - r0);
+    const int klo = LOWER ? 0 : r0 + bs, khi = LOWER ? r0 : n;
+    // ---- off-diagonal contributions: row r of the block; its S threads take
+    // interleaved columns (klo + sg + S m), so a row's threads read
+    // consecutive factor entries and consecutive rows of the LDS slice
+    T acc[CW];
+#pragma unroll
+    for (int c = 0; c < CW; ++c) acc[c] = T(0);
+    if (r < bs && klo < khi) {
+      const T* __restrict__ Lr = LU + (int64_t)(r0 + r) * n;
+      // M entries per thread in flight; the index is clamped so the loads
+      // need no guard, the FMAs are guarded
+      for (int kb = klo + sg; kb < khi; kb += S * M) {
+        T l[M];
+#pragma unroll
+        for (int m = 0; m < M; ++m) l[m] = Lr[min(kb + S * m, khi - 1)];
+#pragma unroll
+        for (int m = 0; m < M; ++m) {
+          const int kk = kb + S * m;
+          if (kk < khi) {
+            const T* xr = xs + (int64_t)kk * CW;
+#pragma unroll
+            for (int c = 0; c < CW; ++c) acc[c] = acc[c] + l[m] * xr[c];
+          }
+        }
+      }
+    }
+    // the S threads of a row are consecutive lanes of one wave
+#pragma unroll
+    for (int m = 1; m < S; m <<= 1)
+#pragma unroll
+      for (int c = 0; c < CW; ++c) acc[c] += __shfl_xor(acc[c], m, 64);
+    if (sg == 0) {
+#pragma unroll
+      for (int c = 0; c < CW; ++c) part[r][c] = acc[c];
+    }
+    __syncthreads();
+    // ---- diagonal block: wave w solves columns w, w + NWV, ... (lane = row)
+    if (wave < CW) {
+      const int i = r0 + lane;
+      T row[GB];
+      const T* lr = LU + (int64_t)(lane < bs ? i : r0) * n + r0;
+#pragma unroll
+      for (int q = 0; q < GB; ++q) row[q] = (lane < bs && q < bs) ? lr[q] : T(0);
+      T x[CPW];
+#pragma unroll
+      for (int j = 0; j < CPW; ++j) {
+        const int c = wave + j * NWV;
+        x[j] = (lane < bs && c < CW) ? xs[(int64_t)i * CW + c] - part[lane][c] : T(0);
+      }
+      if (LOWER) {
+#pragma unroll
+        for (int q = 0; q < GB; ++q) {
+          const T f = lane > q ? row[q] : T(0);
+#pragma unroll
+          for (int j = 0; j < CPW; ++j) x[j] = x[j] - f * readlane<T>(x[j], q);
+        }
+      } else {
+        T d = T(1);
+#pragma unroll
+        for (int q = 0; q < GB; ++q) d = (q == lane && lane < bs) ? row[q] : d;
+        // one division per lane, in parallel; the row's scaling at its step is
+        // then a multiply (within one rounding of lu_solve's division)
+        const T dinv = T(1) / d;
+#pragma unroll
+        for (int q = GB - 1; q >= 0; --q) {
+          if (lane == q) {
+#pragma unroll
+            for (int j = 0; j < CPW; ++j) x[j] = x[j] * dinv;
+          }
+          const T f = lane < q ? row[q] : T(0);
+#pragma unroll
+          for (int j = 0; j < CPW; ++j) x[j] = x[j] - f * readlane<T>(x[j], q);
+        }
+      }
+      if (lane < bs) {
+#pragma unroll
+        for (int j = 0; j < CPW; ++j) {
+          const int c = wave + j * NWV;
+          if (c < CW) xs[(int64_t)i * CW + c] = x[j];
+        }
+      }
+    }
+    __syncthreads();
+  }
+  for (int i = t; i < n; i += NT) {
+    T* xr = X + (int64_t)i * nrhs + c0;
+#pragma unroll
+    for (int c = 0; c < CW; ++c)
+      if (c < nc) xr[c] = xs[(int64_t)i * CW + c];
+  }
 }
 
 // Permutation from LAPACK-style pivots: one wave per matrix replays the swaps
@@ -191,27 +194,97 @@ __global__ __launch_bounds__(64) void ipiv_to_perm_kernel(int n, const int32_t* 
 
 }  // namespace
 
+// Dynamic LDS a sweep may use: the device's per-workgroup maximum less room
+// for the static arrays, at most 128 KiB, at least the 64 KiB every launch may
+// use without opting in.
+size_t lds_budget() {
+  static const size_t b = [] {
+    int dev = 0, v = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+      return (size_t)64 * 1024;
+    const size_t avail = v > 8 * 1024 ? (size_t)v - 8 * 1024 : 0;
+    return std::max<size_t>(64 * 1024, std::min<size_t>(avail, 128 * 1024));
+  }();
+  return b;
+}
+
+// Right-hand-side columns per workgroup: the widest of 8, 4, 2, 1 whose n x CW
+// slice fits in LDS and that still gives at least one workgroup per CU (each
+// workgroup streams its matrix's whole triangle, so few wide workgroups leave
+// most CUs idle). NUMPP_HIP_GETRS_CW = 1, 2, 4 or 8 caps it.
+template <class T>
+int chunk_width(int64_t count, int n, int nrhs) {
+  static const int cap_env = [] {
+    const char* e = std::getenv("NUMPP_HIP_GETRS_CW");
+    const int v = e ? std::atoi(e) : 8;
+    return (v == 1 || v == 2 || v == 4) ? v : 8;
+  }();
+  const size_t cap = lds_budget();
+  const int64_t want = cu_count() > 0 ? cu_count() : 256;
+  int cw = cap_env;
+  while (cw > 1 && ((size_t)n * cw * sizeof(T) > cap || count * ceil_div(nrhs, cw) < want)) cw /= 2;
+  return cw;
+}
+
+// Launches one sweep; dynamic LDS above the default 64 KiB is enabled once per
+// instantiation.
+template <class T, bool LOWER, int NT, int CW>
+int launch_sweep_cw(dim3 grid, size_t shm, hipStream_t s, int n, int nrhs, const T* LU, int64_t strideA,
+                    const int32_t* perm, const T* B, int64_t strideB, T* X, int64_t strideX, const int32_t* info,
+                    const int32_t* mask) {
+  auto* k = sweep_kernel<T, LOWER, NT, CW>;
+  if (shm > 64 * 1024) {
+    static const hipError_t once = hipFuncSetAttribute(
+        reinterpret_cast<const void*>(k), hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_budget());
+    NPPH_HIP(once);
+  }
+  hipLaunchKernelGGL(k, grid, dim3(NT), shm, s, n, nrhs, LU, strideA, perm, B, strideB, X, strideX, info, mask);
+  return NPPH_OK;
+}
+
+template <class T, bool LOWER, int NT>
+int launch_sweep(int cw, dim3 grid, size_t shm, hipStream_t s, int n, int nrhs, const T* LU, int64_t strideA,
+                 const int32_t* perm, const T* B, int64_t strideB, T* X, int64_t strideX, const int32_t* info,
+                 const int32_t* mask) {
+  if (cw == 8)
+    return launch_sweep_cw<T, LOWER, NT, 8>(grid, shm, s, n, nrhs, LU, strideA, perm, B, strideB, X, strideX, info,
+                                            mask);
+  if (cw == 4)
+    return launch_sweep_cw<T, LOWER, NT, 4>(grid, shm, s, n, nrhs, LU, strideA, perm, B, strideB, X, strideX, info,
+                                            mask);
+  if (cw == 2)
+    return launch_sweep_cw<T, LOWER, NT, 2>(grid, shm, s, n, nrhs, LU, strideA, perm, B, strideB, X, strideX, info,
+                                            mask);
+  return launch_sweep_cw<T, LOWER, NT, 1>(grid, shm, s, n, nrhs, LU, strideA, perm, B, strideB, X, strideX, info,
+                                          mask);
+}
+
 template <class T>
 int blocked_getrs_perm(int64_t count, int n, int nrhs, const T* LU, int64_t strideA, const int32_t* perm,
                        const T* B, int64_t strideB, T* X, int64_t strideX, const int32_t* info, const int32_t* mask,
                        hipStream_t s) {
   if (count <= 0 || n <= 0 || nrhs <= 0) return NPPH_OK;
   if (B == X) { set_error("getrs: B and X must not alias"); return NPPH_ERR_INVALID; }
-  if (count > 65535) { set_error("getrs: count too large"); return NPPH_ERR_INVALID; }
-  const int nblk = (int)ceil_div(n, GB), nz = (int)ceil_div(nrhs, GNR);
-  const int64_t nflag = count * nz * (int64_t)nblk;
-  int* flags;
-  NPPH_TRY(workspace(kWsFlags, sizeof(int) * (size_t)(2 * nflag + 1), (void**)&flags));
-  int* err = flags + 2 * nflag;
-  hipLaunchKernelGGL(clear_flags_kernel, dim3((unsigned)ceil_div(2 * nflag + 1, 256)), dim3(256), 0, s, flags,
-                     2 * nflag + 1);
-  dim3 grid((unsigned)count, (unsigned)nblk, (unsigned)nz);
+  if (count > 2147483647LL) { set_error("getrs: count too large"); return NPPH_ERR_INVALID; }
+  const int cw = chunk_width<T>(count, n, nrhs);
+  const int nz = (int)ceil_div(nrhs, cw);
+  if (nz > 65535) { set_error("getrs: nrhs too large"); return NPPH_ERR_INVALID; }
+  const size_t shm = (size_t)n * cw * sizeof(T);
+  dim3 grid((unsigned)count, (unsigned)nz);
   ProfScope ps(kProfGetrs, 2.0 * count * n * n * nrhs,
                (double)count * (sizeof(T) * (double)n * n + 3.0 * sizeof(T) * n * nrhs), s);
-  hipLaunchKernelGGL((sweep_kernel<T, true>), grid, dim3(GT), 0, s, n, nrhs, LU, strideA, perm, B, strideB, X,
-                     strideX, flags, err, info, mask);
-  hipLaunchKernelGGL((sweep_kernel<T, false>), grid, dim3(GT), 0, s, n, nrhs, LU, strideA, perm, B, strideB, X,
-                     strideX, flags + nflag, err, info, mask);
+  if (n <= 256) {
+    NPPH_TRY((launch_sweep<T, true, 256>(cw, grid, shm, s, n, nrhs, LU, strideA, perm, B, strideB, X, strideX, info,
+                                         mask)));
+    NPPH_TRY((launch_sweep<T, false, 256>(cw, grid, shm, s, n, nrhs, LU, strideA, perm, B, strideB, X, strideX, info,
+                                          mask)));
+  } else {
+    NPPH_TRY((launch_sweep<T, true, 512>(cw, grid, shm, s, n, nrhs, LU, strideA, perm, B, strideB, X, strideX, info,
+                                         mask)));
+    NPPH_TRY((launch_sweep<T, false, 512>(cw, grid, shm, s, n, nrhs, LU, strideA, perm, B, strideB, X, strideX, info,
+                                          mask)));
+  }
   NPPH_HIP(hipGetLastError());
   return NPPH_OK;
 }

@@ -16,9 +16,11 @@
 // sum_{j<i} n_j^2, its B and X at sum_{j<i} n_j*nrhs_j, its berr at
 // sum_{j<i} nrhs_j; all row-major.
 //
-// The FP64 execution (pack, per-bucket LU + solve, unpack) is captured once into
-// a hipGraph and replayed; the graph is re-captured when the data pointers or a
-// workspace address change.
+// Every precision reports the normwise backward error of each result (FP64:
+// LU, getrs and one residual; FP32/BF16: refinement). The FP64 execution (pack,
+// per-bucket LU + solve + report, unpack) is captured once into a hipGraph and
+// replayed; the graph is re-captured when the data pointers or a workspace
+// address change.
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -36,8 +38,6 @@
 
 namespace npph {
 
-int gesv_dev_impl(int64_t count, int n, int nrhs, double* A, int64_t strideA, double* B, int64_t strideB,
-                  int32_t* ipiv, int32_t* info, hipStream_t s);
 int refined_impl(const npph_refine_opts& o, int64_t count, int n, int nrhs, const double* A, const double* B,
                  double* X, double* berr, int32_t* iters_out, int32_t* info_out, hipStream_t s);
 
@@ -53,7 +53,6 @@ struct Bucket {
   double* berr = nullptr;
   int32_t* iters = nullptr;
   int32_t* info = nullptr;
-  int32_t* ipiv = nullptr;
 };
 
 struct Plan {
@@ -214,11 +213,7 @@ int tune_cost(const npph_refine_opts& o, int N, int R, double* us, hipStream_t s
   for (int rep = 0; rep < 3 && st == NPPH_OK; ++rep) {
     NPPH_HIP(hipMemcpyAsync(A, A0, sizeof(double) * total, hipMemcpyDeviceToDevice, s));
     NPPH_HIP(hipEventRecord(e0, s));
-    if (o.precision == NPPH_PREC_FP64) {
-      st = gesv_dev_impl(cnt, N, R, A, (int64_t)N * N, B, (int64_t)N * R, nullptr, nullptr, s);
-    } else {
-      st = refined_impl(o, cnt, N, R, A, B, X, E, I, F, s);
-    }
+    st = refined_impl(o, cnt, N, R, A, B, X, E, I, F, s);
     NPPH_HIP(hipEventRecord(e1, s));
     NPPH_HIP(hipEventSynchronize(e1));
     float ms = 0.f;
@@ -290,7 +285,6 @@ int plan_build(Plan* p, int autotune, hipStream_t s) {
     NPPH_TRY(dalloc(p, &b.berr, c * b.R));
     NPPH_TRY(dalloc(p, &b.iters, c));
     NPPH_TRY(dalloc(p, &b.info, c));
-    NPPH_TRY(dalloc(p, &b.ipiv, c * b.N));
     NPPH_HIP(hipMemcpyAsync(b.d_ids, b.ids.data(), sizeof(int32_t) * c, hipMemcpyHostToDevice, s));
     p->buckets.push_back(std::move(b));
   }
@@ -301,19 +295,16 @@ int plan_build(Plan* p, int autotune, hipStream_t s) {
 // One execution without graph capture.
 int run_plan(Plan* p, const double* A, const double* B, double* X, double* berr, int32_t* iters, int32_t* info,
              hipStream_t s) {
-  const bool fp64 = p->opts.precision == NPPH_PREC_FP64;
   for (auto& b : p->buckets) {
     const int64_t c = (int64_t)b.ids.size();
     hipLaunchKernelGGL(pack_kernel, dim3((unsigned)c), dim3(256), 0, s, b.N, b.R, b.d_ids, p->d_n, p->d_nrhs,
-                       p->d_aoff, p->d_boff, A, B, b.A, fp64 ? b.X : b.B);
+                       p->d_aoff, p->d_boff, A, B, b.A, b.B);
     NPPH_HIP(hipGetLastError());
-    if (fp64) {
-      NPPH_TRY(gesv_dev_impl(c, b.N, b.R, b.A, (int64_t)b.N * b.N, b.X, (int64_t)b.N * b.R, b.ipiv, b.info, s));
-    } else {
-      NPPH_TRY(refined_impl(p->opts, c, b.N, b.R, b.A, b.B, b.X, b.berr, b.iters, b.info, s));
-    }
+    // FP64 runs LU + getrs + the backward-error report (iters = 0); the other
+    // precisions refine.
+    NPPH_TRY(refined_impl(p->opts, c, b.N, b.R, b.A, b.B, b.X, b.berr, b.iters, b.info, s));
     hipLaunchKernelGGL(unpack_kernel, dim3((unsigned)c), dim3(256), 0, s, b.N, b.R, b.d_ids, p->d_n, p->d_nrhs,
-                       p->d_boff, p->d_coff, b.X, b.info, fp64 ? nullptr : b.berr, fp64 ? nullptr : b.iters, X, info,
+                       p->d_boff, p->d_coff, b.X, b.info, b.berr, b.iters, X, info,
                        berr, iters);
     NPPH_HIP(hipGetLastError());
   }

@@ -29,10 +29,17 @@ def make(seed, count, nlo=8, nhi=256, rmax=16, dominant=True):
 def test_fp64_uneven_vs_oracle(hip):
     mats, rhss = make(4, 60)
     xs, berr, iters, info = vbatch.solve_uneven(mats, rhss, "fp64")
-    assert np.all(info == 0)
+    assert np.all(info == 0) and np.all(iters == 0)
+    off = 0
     for a, b, x in zip(mats, rhss, xs):
         xo = oracle.solve(a, b)
         assert np.abs(x - xo).max() <= 1e-12 * np.abs(xo).max()
+        # the reported backward error (FP64 residual) agrees with the oracle's
+        # Dot2 one to the FP64 residual's own rounding (<= n u of the scale)
+        eta = oracle.backward_error(a, b, x)
+        rep = berr[off:off + b.shape[1]]
+        assert np.all(rep >= 0) and np.all(np.abs(rep - eta) <= 1e-14)
+        off += b.shape[1]
 
 
 def test_general_matrices_pivoting(hip):
