@@ -25,7 +25,7 @@ namespace npph {
 
 namespace {
 
-constexpr int BM = 128, BN = 128, BK = 16, NT = 256;
+constexpr int BM = 128, BK = 16, NT = 256;
 constexpr int PAD = 4;
 
 template <class T>
@@ -61,9 +61,12 @@ __device__ __forceinline__ void tile_coords(int bid, int tiles_m, int tiles_n, i
   tn = r / gsize;
 }
 
-template <class T>
-__global__ __launch_bounds__(NT) void gemm_kernel(GemmArgs<T> p) {
+// BN = 128 (wave tile 64 x 64) or 64 (wave tile 64 x 32, for narrow updates).
+template <class T, int BN>
+__global__ __launch_bounds__(NT, 2) void gemm_kernel(GemmArgs<T> p) {
   typedef typename Mfma<T>::acc_t acc_t;
+  constexpr int JT = BN / 32;      // 16-column MFMA tiles per wave
+  constexpr int BE = BN / 16;      // B elements per thread per slice
   __shared__ T As[2][BK][BM + PAD];
   __shared__ T Bs[2][BK][BN + PAD];
 
@@ -80,10 +83,10 @@ __global__ __launch_bounds__(NT) void gemm_kernel(GemmArgs<T> p) {
 
   // Global-load mapping: 8 elements per thread per operand.
   const int a_row = tid >> 1, a_k = (tid & 1) * 8;    // A tile: BM rows x BK
-  const int b_k = tid >> 4, b_col = (tid & 15) * 8;   // B tile: BK rows x BN
+  const int b_k = tid >> 4, b_col = (tid & 15) * BE;  // B tile: BK rows x BN
   const bool mn_full = (m0 + BM <= p.M) && (n0 + BN <= p.N);
 
-  T ra[8], rb[8];
+  T ra[8], rb[BE];
   auto load = [&](int64_t k0) {
     const bool full = mn_full && (k0 + BK <= p.K) && p.vec;
     const int64_t gr = m0 + a_row, gk = k0 + a_k;
@@ -95,39 +98,44 @@ __global__ __launch_bounds__(NT) void gemm_kernel(GemmArgs<T> p) {
 #pragma unroll
         for (int i = 0; i < 8; i += 2) {
           double2 va = *reinterpret_cast<const double2*>(pa + i);
-          double2 vb = *reinterpret_cast<const double2*>(pb + i);
           ra[i] = va.x; ra[i + 1] = va.y;
+        }
+#pragma unroll
+        for (int i = 0; i < BE; i += 2) {
+          double2 vb = *reinterpret_cast<const double2*>(pb + i);
           rb[i] = vb.x; rb[i + 1] = vb.y;
         }
       } else {
 #pragma unroll
         for (int i = 0; i < 8; i += 4) {
           float4 va = *reinterpret_cast<const float4*>(pa + i);
-          float4 vb = *reinterpret_cast<const float4*>(pb + i);
           ra[i] = va.x; ra[i + 1] = va.y; ra[i + 2] = va.z; ra[i + 3] = va.w;
+        }
+#pragma unroll
+        for (int i = 0; i < BE; i += 4) {
+          float4 vb = *reinterpret_cast<const float4*>(pb + i);
           rb[i] = vb.x; rb[i + 1] = vb.y; rb[i + 2] = vb.z; rb[i + 3] = vb.w;
         }
       }
     } else {
 #pragma unroll
-      for (int i = 0; i < 8; ++i) {
-        ra[i] = (gr < p.M && gk + i < p.K) ? A[gr * p.lda + gk + i] : T(0);
-        rb[i] = (bk < p.K && bc + i < p.N) ? B[bk * p.ldb + bc + i] : T(0);
-      }
+      for (int i = 0; i < 8; ++i) ra[i] = (gr < p.M && gk + i < p.K) ? A[gr * p.lda + gk + i] : T(0);
+#pragma unroll
+      for (int i = 0; i < BE; ++i) rb[i] = (bk < p.K && bc + i < p.N) ? B[bk * p.ldb + bc + i] : T(0);
     }
   };
   auto store = [&](int buf) {
 #pragma unroll
     for (int i = 0; i < 8; ++i) As[buf][a_k + i][a_row] = ra[i];
 #pragma unroll
-    for (int i = 0; i < 8; ++i) Bs[buf][b_k][b_col + i] = rb[i];
+    for (int i = 0; i < BE; ++i) Bs[buf][b_k][b_col + i] = rb[i];
   };
 
-  acc_t acc[4][4];
+  acc_t acc[4][JT];
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
+    for (int j = 0; j < JT; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
 
   const int kt_total = (int)((p.K + BK - 1) / BK);
   load(0);
@@ -139,15 +147,15 @@ __global__ __launch_bounds__(NT) void gemm_kernel(GemmArgs<T> p) {
     if (kt + 1 < kt_total) load((int64_t)(kt + 1) * BK);
 #pragma unroll
     for (int kk = 0; kk < BK; kk += 4) {
-      T af[4], bf[4];
+      T af[4], bf[JT];
 #pragma unroll
       for (int i = 0; i < 4; ++i) af[i] = As[cur][kk + lk][wm * 64 + i * 16 + lr];
 #pragma unroll
-      for (int j = 0; j < 4; ++j) bf[j] = Bs[cur][kk + lk][wn * 64 + j * 16 + lr];
+      for (int j = 0; j < JT; ++j) bf[j] = Bs[cur][kk + lk][wn * (BN / 2) + j * 16 + lr];
 #pragma unroll
       for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = Mfma<T>::run(af[i], bf[j], acc[i][j]);
+        for (int j = 0; j < JT; ++j) acc[i][j] = Mfma<T>::run(af[i], bf[j], acc[i][j]);
     }
     if (kt + 1 < kt_total) store(cur ^ 1);
     __syncthreads();
@@ -163,8 +171,8 @@ __global__ __launch_bounds__(NT) void gemm_kernel(GemmArgs<T> p) {
       if (row >= p.M) continue;
       T* crow = C + row * p.ldc;
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const int64_t col = n0 + wn * 64 + j * 16 + lr;
+      for (int j = 0; j < JT; ++j) {
+        const int64_t col = n0 + wn * (BN / 2) + j * 16 + lr;
         if (col < p.N) {
           T r = p.alpha * acc[i][j][v];
           if (!beta0) r += p.beta * crow[col];
@@ -197,6 +205,10 @@ int gemm_launch(GemmArgs<T> p, int64_t batch, hipStream_t s) {
     set_error("gemm: leading dimension too small");
     return NPPH_ERR_INVALID;
   }
+  // 64-wide column tiles for narrow products (the lowest recursion levels),
+  // where a 128-wide tile would leave half of its MFMA columns empty
+  const bool narrow = p.N <= 64;
+  const int BN = narrow ? 64 : 128;
   p.tiles_m = (int)ceil_div(p.M, BM);
   p.tiles_n = (int)ceil_div(p.N, BN);
   p.vec = vec_ok(p) ? 1 : 0;
@@ -205,7 +217,8 @@ int gemm_launch(GemmArgs<T> p, int64_t batch, hipStream_t s) {
   const double es = sizeof(T), fl = 2.0 * p.M * p.N * p.K * batch;
   const double by = es * batch * ((double)p.M * p.K + (double)p.K * p.N + (p.beta == T(0) ? 1.0 : 2.0) * p.M * p.N);
   ProfScope ps(sizeof(T) == 8 ? kProfGemmF64 : kProfGemmF32, fl, by, s);
-  hipLaunchKernelGGL(gemm_kernel<T>, grid, dim3(NT), 0, s, p);
+  if (narrow) hipLaunchKernelGGL((gemm_kernel<T, 64>), grid, dim3(NT), 0, s, p);
+  else hipLaunchKernelGGL((gemm_kernel<T, 128>), grid, dim3(NT), 0, s, p);
   NPPH_HIP(hipGetLastError());
   return NPPH_OK;
 }

@@ -637,7 +637,7 @@ int groups_for(int64_t count, int n) {
     const char* e = std::getenv("NUMPP_HIP_LU_GROUPS");
     return e ? std::atoi(e) : -1;
   }();
-  int G = env > 0 ? env : (count >= 16 ? kGroups : (count >= 4 ? 2 : 1));
+  int G = env > 0 ? env : (count >= 4 ? 2 : 1);
   if (n < 256) G = 1;
   return std::max(1, std::min<int>(G, (int)std::min<int64_t>(count, kGroups)));
 }
