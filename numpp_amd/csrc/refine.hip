@@ -142,15 +142,17 @@ __global__ void count_sweep_kernel(int64_t count, const int32_t* active, int32_t
 }
 
 // R = B - A X in two passes. Pass 1 (residual_partial_kernel): one
-// workgroup per (16-row block, matrix, 512-column chunk x 8-RHS chunk) issues
-// its A loads (4 rows per wave, two adjacent columns per lane in each of four
-// 128-column slices, 16-byte loads) before staging its X chunk in LDS, so the
-// A traffic is in flight while X arrives; its 32 partial sums per lane are
+// workgroup per (64-row block, matrix, 512-column chunk x 8-RHS chunk) stages
+// its X chunk in LDS once and walks its rows in four 16-row passes (4 rows per
+// wave, two adjacent columns per lane in each of four 128-column slices,
+// 16-byte loads); the first pass's A loads are issued before the X staging, so
+// that traffic is in flight while X arrives; its 32 partial sums per lane are
 // combined with an xor step and a halving butterfly and written as one
 // partial per (row, column, chunk). Pass 2 (residual_finish_kernel) adds the
 // partials in chunk order (deterministic), forms r = b - sum and the per-column
 // max |r|. The block of matrix rows with index 0 also reports max |x| per column.
 constexpr int RES_RPW = 4, RES_ROWS = 4 * RES_RPW, RES_JC = 512, RES_NC = 8;
+constexpr int RES_PASS = 4;  // 16-row passes per workgroup over one staged X chunk
 __global__ __launch_bounds__(256) void residual_partial_kernel(int n, int nrhs, int nchunk, const double* Ab,
                                                                const double* Xb, double* part, double* xmax,
                                                                const int32_t* active) {
@@ -163,31 +165,34 @@ __global__ __launch_bounds__(256) void residual_partial_kernel(int n, int nrhs, 
   const double* __restrict__ A = Ab + mat * (int64_t)n * n;
   const double* __restrict__ X = Xb + mat * (int64_t)n * nrhs;
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-  const int row0 = blockIdx.x * RES_ROWS + wave * RES_RPW;
   __shared__ double xs[RES_JC][RES_NC];
   const bool even = (n & 1) == 0;
   double a0[RES_JC / 128][RES_RPW], a1[RES_JC / 128][RES_RPW];
+  auto load_a = [&](int row0) {
 #pragma unroll
-  for (int sub = 0; sub < RES_JC / 128; ++sub) {
-    const int j0 = jc + sub * 128 + 2 * lane;
+    for (int sub = 0; sub < RES_JC / 128; ++sub) {
+      const int j0 = jc + sub * 128 + 2 * lane;
 #pragma unroll
-    for (int r = 0; r < RES_RPW; ++r) {
-      const int row = row0 + r;
-      a0[sub][r] = 0.0;
-      a1[sub][r] = 0.0;
-      if (row < n && j0 < n) {
-        const double* ar = A + (int64_t)row * n;
-        if (even) {
-          const double2 v = *reinterpret_cast<const double2*>(ar + j0);
-          a0[sub][r] = v.x;
-          a1[sub][r] = v.y;
-        } else {
-          a0[sub][r] = ar[j0];
-          a1[sub][r] = j0 + 1 < n ? ar[j0 + 1] : 0.0;
+      for (int r = 0; r < RES_RPW; ++r) {
+        const int row = row0 + r;
+        a0[sub][r] = 0.0;
+        a1[sub][r] = 0.0;
+        if (row < n && j0 < n) {
+          const double* ar = A + (int64_t)row * n;
+          if (even) {
+            const double2 v = *reinterpret_cast<const double2*>(ar + j0);
+            a0[sub][r] = v.x;
+            a1[sub][r] = v.y;
+          } else {
+            a0[sub][r] = ar[j0];
+            a1[sub][r] = j0 + 1 < n ? ar[j0 + 1] : 0.0;
+          }
         }
       }
     }
-  }
+  };
+  const int rowbase = blockIdx.x * (RES_ROWS * RES_PASS) + wave * RES_RPW;
+  load_a(rowbase);
   double xm = 0.0;
 #pragma unroll
   for (int q = 0; q < (RES_JC * RES_NC) / 256; ++q) {
@@ -202,41 +207,48 @@ __global__ __launch_bounds__(256) void residual_partial_kernel(int n, int nrhs, 
     if (cc < nc) atomic_max_nonneg(xmax + mat * nrhs + c0 + cc, xm);
   }
   __syncthreads();
-  double acc[RES_RPW][RES_NC];
-#pragma unroll
-  for (int r = 0; r < RES_RPW; ++r)
-#pragma unroll
-    for (int c = 0; c < RES_NC; ++c) acc[r][c] = 0.0;
-#pragma unroll
-  for (int sub = 0; sub < RES_JC / 128; ++sub) {
-    const int jl = sub * 128 + 2 * lane;
-#pragma unroll
-    for (int c = 0; c < RES_NC; ++c) {
-      const double x0 = xs[jl][c], x1 = xs[jl + 1][c];
-#pragma unroll
-      for (int r = 0; r < RES_RPW; ++r) acc[r][c] = fma(a1[sub][r], x1, fma(a0[sub][r], x0, acc[r][c]));
+  for (int pass = 0; pass < RES_PASS; ++pass) {
+    const int row0 = rowbase + pass * RES_ROWS;
+    if (pass > 0) {
+      if (row0 - wave * RES_RPW >= n) break;  // uniform over the workgroup
+      load_a(row0);
     }
-  }
-  // 32 values per lane: fold the two half-waves, then a halving butterfly.
-  double v[RES_RPW * RES_NC];
+    double acc[RES_RPW][RES_NC];
 #pragma unroll
-  for (int r = 0; r < RES_RPW; ++r)
+    for (int r = 0; r < RES_RPW; ++r)
 #pragma unroll
-    for (int c = 0; c < RES_NC; ++c) v[r * RES_NC + c] = acc[r][c] + __shfl_xor(acc[r][c], 32, 64);
+      for (int c = 0; c < RES_NC; ++c) acc[r][c] = 0.0;
 #pragma unroll
-  for (int h = 16; h >= 1; h >>= 1) {
-    const bool up = (lane & h) != 0;
+    for (int sub = 0; sub < RES_JC / 128; ++sub) {
+      const int jl = sub * 128 + 2 * lane;
 #pragma unroll
-    for (int i = 0; i < h; ++i) {
-      const double send = up ? v[i] : v[i + h];
-      const double keep = up ? v[i + h] : v[i];
-      v[i] = keep + __shfl_xor(send, h, 64);
+      for (int c = 0; c < RES_NC; ++c) {
+        const double x0 = xs[jl][c], x1 = xs[jl + 1][c];
+#pragma unroll
+        for (int r = 0; r < RES_RPW; ++r) acc[r][c] = fma(a1[sub][r], x1, fma(a0[sub][r], x0, acc[r][c]));
+      }
     }
+    // 32 values per lane: fold the two half-waves, then a halving butterfly.
+    double v[RES_RPW * RES_NC];
+#pragma unroll
+    for (int r = 0; r < RES_RPW; ++r)
+#pragma unroll
+      for (int c = 0; c < RES_NC; ++c) v[r * RES_NC + c] = acc[r][c] + __shfl_xor(acc[r][c], 32, 64);
+#pragma unroll
+    for (int h = 16; h >= 1; h >>= 1) {
+      const bool up = (lane & h) != 0;
+#pragma unroll
+      for (int i = 0; i < h; ++i) {
+        const double send = up ? v[i] : v[i + h];
+        const double keep = up ? v[i + h] : v[i];
+        v[i] = keep + __shfl_xor(send, h, 64);
+      }
+    }
+    const int r = (lane & 31) / RES_NC, c = lane % RES_NC;
+    const int row = row0 + r;
+    if (lane < 32 && row < n && c < nc)
+      part[((mat * nchunk + ch) * (int64_t)n + row) * nrhs + c0 + c] = v[0];
   }
-  const int r = (lane & 31) / RES_NC, c = lane % RES_NC;
-  const int row = row0 + r;
-  if (lane < 32 && row < n && c < nc)
-    part[((mat * nchunk + ch) * (int64_t)n + row) * nrhs + c0 + c] = v[0];
 }
 
 __global__ void residual_finish_kernel(int64_t count, int n, int nrhs, int nchunk, const double* part,
@@ -399,7 +411,7 @@ int residual_launch(int64_t count, int n, int nrhs, const double* A, const doubl
   const int nchunk = (int)ceil_div(n, RES_JC), nz = (int)ceil_div(nrhs, RES_NC);
   double* part;
   NPPH_TRY(workspace(kWsPart, sizeof(double) * (size_t)count * nchunk * n * nrhs, (void**)&part));
-  dim3 grid((unsigned)ceil_div(n, RES_ROWS), (unsigned)count, (unsigned)(nchunk * nz));
+  dim3 grid((unsigned)ceil_div(n, RES_ROWS * RES_PASS), (unsigned)count, (unsigned)(nchunk * nz));
   ProfScope ps(kProfResidual, 2.0 * count * n * n * nrhs, 8.0 * count * ((double)n * n + 3.0 * n * nrhs), s);
   hipLaunchKernelGGL(residual_partial_kernel, grid, dim3(256), 0, s, n, nrhs, nchunk, A, X, part, xmax, active);
   hipLaunchKernelGGL(residual_finish_kernel, g1(count * n * nrhs), dim3(256), 0, s, count, n, nrhs, nchunk, part, B,
