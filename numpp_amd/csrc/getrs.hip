@@ -130,9 +130,11 @@ __global__ __launch_bounds__(NT) void sweep_kernel(int n, int nrhs, const T* LUb
         const int c = wave + j * NWV;
         x[j] = (lane < bs && c < CW) ? xs[(int64_t)i * CW + c] - part[lane][c] : T(0);
       }
+      // steps q >= bs touch only padding rows: skipped (bs is uniform)
       if (LOWER) {
 #pragma unroll
         for (int q = 0; q < GB; ++q) {
+          if (q + 1 >= bs) break;
           const T f = lane > q ? row[q] : T(0);
 #pragma unroll
           for (int j = 0; j < CPW; ++j) x[j] = x[j] - f * readlane<T>(x[j], q);
@@ -146,6 +148,7 @@ __global__ __launch_bounds__(NT) void sweep_kernel(int n, int nrhs, const T* LUb
         const T dinv = T(1) / d;
 #pragma unroll
         for (int q = GB - 1; q >= 0; --q) {
+          if (q >= bs) continue;
           if (lane == q) {
 #pragma unroll
             for (int j = 0; j < CPW; ++j) x[j] = x[j] * dinv;

@@ -632,12 +632,15 @@ int max_blocked_n() { return 8 * PT; }
 namespace {
 
 // Sub-batches per factorisation (NUMPP_HIP_LU_GROUPS overrides).
-int groups_for(int64_t count, int n) {
+int groups_for(int64_t count, int n, bool f32) {
   static const int env = [] {
     const char* e = std::getenv("NUMPP_HIP_LU_GROUPS");
     return e ? std::atoi(e) : -1;
   }();
-  int G = env > 0 ? env : (count >= 4 ? 2 : 1);
+  // measured on 64 x 2048^2: FP32 factorisations gain nothing from a second
+  // sub-batch (its leaves and updates already fill the device between them),
+  // FP64 ones gain about 1%
+  int G = env > 0 ? env : ((count >= 4 && !f32) ? 2 : 1);
   if (n < 256) G = 1;
   return std::max(1, std::min<int>(G, (int)std::min<int64_t>(count, kGroups)));
 }
@@ -712,7 +715,7 @@ int blocked_getrf_perm(int64_t count, int n, T* A, int64_t strideA, int32_t* ipi
   // or a caller that is about to capture gets a plain issue on its stream.
   if (ws_current() != nullptr || plain_issue() || stream_capturing(s))
     return getrf_group<T>(count, n, A, strideA, ipiv, perm, info, bf16_terms, mask, s);
-  const int G = groups_for(count, n);
+  const int G = groups_for(count, n, sizeof(T) == 4);
   // Per-launch profiling needs an eager issue (a replayed graph has no
   // per-kernel events).
   if (!graphs_enabled() || prof_enabled())
