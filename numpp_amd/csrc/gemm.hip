@@ -193,7 +193,7 @@ bool vec_ok(const GemmArgs<T>& p) {
 }  // namespace
 
 template <class T>
-int gemm_launch(GemmArgs<T> p, int64_t batch, hipStream_t s) {
+int gemm_launch(GemmArgs<T> p, int64_t batch, hipStream_t s, int prof) {
   if (p.M <= 0 || p.N <= 0 || batch <= 0) return NPPH_OK;
   if (p.K <= 0) {
     // C = beta * C handled by a K=1 pass with a zero A column is wasteful; the
@@ -216,15 +216,16 @@ int gemm_launch(GemmArgs<T> p, int64_t batch, hipStream_t s) {
   dim3 grid((unsigned)(p.tiles_m * p.tiles_n), (unsigned)batch);
   const double es = sizeof(T), fl = 2.0 * p.M * p.N * p.K * batch;
   const double by = es * batch * ((double)p.M * p.K + (double)p.K * p.N + (p.beta == T(0) ? 1.0 : 2.0) * p.M * p.N);
-  ProfScope ps(sizeof(T) == 8 ? kProfGemmF64 : kProfGemmF32, fl, by, s);
+  const int kind = prof == -1 ? (sizeof(T) == 8 ? kProfGemmF64 : kProfGemmF32) : prof;
+  ProfScope ps(kind, fl, by, s);
   if (narrow) hipLaunchKernelGGL((gemm_kernel<T, 64>), grid, dim3(NT), 0, s, p);
   else hipLaunchKernelGGL((gemm_kernel<T, 128>), grid, dim3(NT), 0, s, p);
   NPPH_HIP(hipGetLastError());
   return NPPH_OK;
 }
 
-template int gemm_launch<double>(GemmArgs<double>, int64_t, hipStream_t);
-template int gemm_launch<float>(GemmArgs<float>, int64_t, hipStream_t);
+template int gemm_launch<double>(GemmArgs<double>, int64_t, hipStream_t, int);
+template int gemm_launch<float>(GemmArgs<float>, int64_t, hipStream_t, int);
 
 }  // namespace npph
 

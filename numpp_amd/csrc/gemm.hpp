@@ -22,8 +22,10 @@ struct GemmArgs {
   int tiles_m, tiles_n, vec;  // filled by gemm_launch
 };
 
+// prof: the npph_prof class the launch is timed under; -1 = by element type,
+// -2 = not timed on its own (the caller's scope covers it).
 template <class T>
-int gemm_launch(GemmArgs<T> p, int64_t batch, hipStream_t s);
+int gemm_launch(GemmArgs<T> p, int64_t batch, hipStream_t s, int prof = -1);
 
 // Split-BF16 update C -= A * B for FP32 matrices (the low-precision LU's
 // trailing update): each operand is split into `terms`-1 BF16 pieces after a

@@ -28,7 +28,7 @@ struct ProfScope {
   void* tok = nullptr;
   hipStream_t s;
   ProfScope(int kind, double flops, double bytes, hipStream_t st) : s(st) {
-    if (prof_enabled()) prof_begin(kind, flops, bytes, st, &tok);
+    if (kind >= 0 && kind < kProfKinds && prof_enabled()) prof_begin(kind, flops, bytes, st, &tok);
   }
   ~ProfScope() {
     if (tok) prof_end(tok, s);

@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "gemm.hpp"
 #include "lu.hpp"
 #include "prof.hpp"
 
@@ -265,80 +266,42 @@ __global__ void residual_finish_kernel(int64_t count, int n, int nrhs, int nchun
   atomic_max_nonneg(rmax + mat * nrhs + (int)(e % nrhs), fabs(res));
 }
 
-// R = B - A X for small orders (n <= 256) in one pass: one workgroup per
-// (matrix, 16-column chunk) stages its X chunk in LDS, then each group of 16
-// lanes walks one row of A (16 consecutive entries per step, four steps of
-// loads in flight), reduces with xor shuffles and writes r, and the workgroup
-// writes the per-column max |r| and max |x| of its matrix (it covers every row,
-// so plain stores replace the atomic maxima of the large-order path).
-constexpr int RS_N = 256, RS_NC = 16;
-__global__ __launch_bounds__(256) void residual_small_kernel(int n, int nrhs, const double* Ab, const double* Xb,
-                                                             const double* Bb, double* Rb, double* rmax,
-                                                             double* xmax, const int32_t* active) {
+// Column maxima of R and X for small orders: one workgroup per system, 16
+// columns at a time, 16 row groups reduced through LDS. The residual itself
+// (R = B - A X) comes from the batched MFMA GEMM (alpha = -1, beta = 1 on a
+// copy of B), which reads A once at full width.
+constexpr int RS_N = 256;
+__global__ __launch_bounds__(256) void colmax_kernel(int n, int nrhs, const double* Rb, const double* Xb,
+                                                     double* rmax, double* xmax, const int32_t* active) {
   const int64_t mat = blockIdx.x;
   if (active && active[mat] != 1) return;
-  const int c0 = (int)blockIdx.y * RS_NC, nc = min(RS_NC, nrhs - c0);
-  const double* __restrict__ A = Ab + mat * (int64_t)n * n;
-  const double* __restrict__ X = Xb + mat * (int64_t)n * nrhs;
-  const double* __restrict__ B = Bb + mat * (int64_t)n * nrhs;
-  double* __restrict__ R = Rb + mat * (int64_t)n * nrhs;
-  const int t = threadIdx.x, g = t >> 4, l = t & 15;
-  __shared__ double xs[RS_N][RS_NC];
-  __shared__ double red_x[16][RS_NC], red_r[16][RS_NC];
-  // X chunk: thread t always holds column t % 16 (256 is a multiple of 16)
-  double xm = 0.0;
-  for (int e = t; e < n * RS_NC; e += 256) {
-    const int j = e / RS_NC, c = e % RS_NC;
-    const double v = c < nc ? X[(int64_t)j * nrhs + c0 + c] : 0.0;
-    xs[j][c] = v;
-    xm = fmax(xm, fabs(v));
-  }
-  __syncthreads();
-  double rm = 0.0;  // lane l: max |r| of column l over this group's rows
-  for (int row = g; row < n; row += 16) {
-    const double* __restrict__ ar = A + (int64_t)row * n;
-    double acc[RS_NC];
-#pragma unroll
-    for (int c = 0; c < RS_NC; ++c) acc[c] = 0.0;
-    for (int j0 = l; j0 < n; j0 += 64) {
-      double a[4];
-#pragma unroll
-      for (int u = 0; u < 4; ++u) a[u] = ar[min(j0 + 16 * u, n - 1)];
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const int j = j0 + 16 * u;
-        if (j < n) {
-#pragma unroll
-          for (int c = 0; c < RS_NC; ++c) acc[c] = fma(a[u], xs[j][c], acc[c]);
-        }
+  const int64_t per = (int64_t)n * nrhs;
+  const double* __restrict__ R = Rb + mat * per;
+  const double* __restrict__ X = Xb + mat * per;
+  const int t = threadIdx.x, c = t & 15, g = t >> 4;
+  __shared__ double red_r[16][16], red_x[16][16];
+  for (int c0 = 0; c0 < nrhs; c0 += 16) {
+    double rm = 0.0, xm = 0.0;
+    if (c0 + c < nrhs) {
+      for (int i = g; i < n; i += 16) {
+        rm = fmax(rm, fabs(R[(int64_t)i * nrhs + c0 + c]));
+        xm = fmax(xm, fabs(X[(int64_t)i * nrhs + c0 + c]));
       }
     }
+    red_r[g][c] = rm;
+    red_x[g][c] = xm;
+    __syncthreads();
+    if (t < 16 && c0 + t < nrhs) {
+      double r = 0.0, x = 0.0;
 #pragma unroll
-    for (int m = 1; m < 16; m <<= 1)
-#pragma unroll
-      for (int c = 0; c < RS_NC; ++c) acc[c] += __shfl_xor(acc[c], m, 64);
-    double mine = acc[0];
-#pragma unroll
-    for (int c = 1; c < RS_NC; ++c) mine = (c == l) ? acc[c] : mine;
-    if (l < nc) {
-      const double r = B[(int64_t)row * nrhs + c0 + l] - mine;
-      R[(int64_t)row * nrhs + c0 + l] = r;
-      rm = fmax(rm, fabs(r));
+      for (int q = 0; q < 16; ++q) {
+        r = fmax(r, red_r[q][t]);
+        x = fmax(x, red_x[q][t]);
+      }
+      rmax[mat * nrhs + c0 + t] = r;
+      xmax[mat * nrhs + c0 + t] = x;
     }
-  }
-  // column maxima over the 16 groups (r) and the 16 thread rows (x)
-  red_r[g][l] = rm;
-  red_x[g][l] = xm;  // thread t holds x column t % 16 = l
-  __syncthreads();
-  if (t < nc) {
-    double r = 0.0, x = 0.0;
-#pragma unroll
-    for (int q = 0; q < 16; ++q) {
-      r = fmax(r, red_r[q][t]);
-      x = fmax(x, red_x[q][t]);
-    }
-    rmax[mat * nrhs + c0 + t] = r;
-    xmax[mat * nrhs + c0 + t] = x;
+    __syncthreads();
   }
 }
 
@@ -401,10 +364,13 @@ inline dim3 g1(int64_t n, int b = 256) { return dim3((unsigned)ceil_div(n, b)); 
 // Residual + per-(system, column) maxima of |r| and |x| for the current X.
 int residual_launch(int64_t count, int n, int nrhs, const double* A, const double* X, const double* B, double* R,
                     double* rmax, double* xmax, const int32_t* active, hipStream_t s) {
-  if (n <= RS_N) {
+  if (n <= RS_N && count <= 65535) {
     ProfScope ps(kProfResidual, 2.0 * count * n * n * nrhs, 8.0 * count * ((double)n * n + 3.0 * n * nrhs), s);
-    hipLaunchKernelGGL(residual_small_kernel, dim3((unsigned)count, (unsigned)ceil_div(nrhs, RS_NC)), dim3(256), 0, s,
-                       n, nrhs, A, X, B, R, rmax, xmax, active);
+    NPPH_HIP(hipMemcpyAsync(R, B, sizeof(double) * (size_t)count * n * nrhs, hipMemcpyDeviceToDevice, s));
+    const int64_t per_b = (int64_t)n * nrhs;
+    GemmArgs<double> g{n, nrhs, n, -1.0, 1.0, A, n, (int64_t)n * n, X, nrhs, per_b, R, nrhs, per_b, 0, 0, 0};
+    NPPH_TRY(gemm_launch<double>(g, count, s, -2));
+    hipLaunchKernelGGL(colmax_kernel, dim3((unsigned)count), dim3(256), 0, s, n, nrhs, R, X, rmax, xmax, active);
     NPPH_HIP(hipGetLastError());
     return NPPH_OK;
   }
