@@ -34,7 +34,9 @@ __global__ __launch_bounds__(NT) void sweep_kernel(int n, int nrhs, const T* LUb
   constexpr int S = NT / GB;       // threads per row in the off-diagonal pass
   constexpr int NWV = NT / 64;     // waves
   constexpr int CPW = (CW + NWV - 1) / NWV;  // columns per wave in the diagonal solve
-  constexpr int M = sizeof(T) == 8 ? 16 : 32;  // factor loads in flight per thread
+  // factor entries per thread and step: deep for long rows (n > 256), short
+  // for small orders, whose ranges are at most 192 columns
+  constexpr int M = NT > 256 ? (sizeof(T) == 8 ? 16 : 32) : 8;
   const int64_t mat = blockIdx.x;
   if ((info && info[mat] != 0) || (mask && mask[mat] == 0)) return;
   const int c0 = (int)blockIdx.y * CW, nc = min(CW, nrhs - c0);
@@ -90,20 +92,20 @@ __global__ __launch_bounds__(NT) void sweep_kernel(int n, int nrhs, const T* LUb
     for (int c = 0; c < CW; ++c) acc[c] = T(0);
     if (r < bs && klo < khi) {
       const T* __restrict__ Lr = LU + (int64_t)(r0 + r) * n;
-      // M entries per thread in flight; the index is clamped so the loads
-      // need no guard, the FMAs are guarded
+      // M entries per thread in flight. Indices past the range are clamped
+      // and their factor entry zeroed, so neither the loads nor the FMAs need
+      // a branch (a clamped slice row is a solved, finite value).
       for (int kb = klo + sg; kb < khi; kb += S * M) {
         T l[M];
 #pragma unroll
         for (int m = 0; m < M; ++m) l[m] = Lr[min(kb + S * m, khi - 1)];
 #pragma unroll
-        for (int m = 0; m < M; ++m) {
-          const int kk = kb + S * m;
-          if (kk < khi) {
-            const T* xr = xs + (int64_t)kk * CW;
+        for (int m = 0; m < M; ++m) l[m] = kb + S * m < khi ? l[m] : T(0);
 #pragma unroll
-            for (int c = 0; c < CW; ++c) acc[c] = acc[c] + l[m] * xr[c];
-          }
+        for (int m = 0; m < M; ++m) {
+          const T* xr = xs + (int64_t)min(kb + S * m, khi - 1) * CW;
+#pragma unroll
+          for (int c = 0; c < CW; ++c) acc[c] = acc[c] + l[m] * xr[c];
         }
       }
     }
@@ -122,8 +124,11 @@ __global__ __launch_bounds__(NT) void sweep_kernel(int n, int nrhs, const T* LUb
       const int i = r0 + lane;
       T row[GB];
       const T* lr = LU + (int64_t)(lane < bs ? i : r0) * n + r0;
+      // clamped unconditional loads (no per-element branches), then masked
 #pragma unroll
-      for (int q = 0; q < GB; ++q) row[q] = (lane < bs && q < bs) ? lr[q] : T(0);
+      for (int q = 0; q < GB; ++q) row[q] = lr[min(q, bs - 1)];
+#pragma unroll
+      for (int q = 0; q < GB; ++q) row[q] = (lane < bs && q < bs) ? row[q] : T(0);
       T x[CPW];
 #pragma unroll
       for (int j = 0; j < CPW; ++j) {
