@@ -46,6 +46,8 @@ __global__ __launch_bounds__(NT) void sweep_kernel(int n, int nrhs, const T* LUb
   extern __shared__ unsigned char smem_raw[];
   T* xs = reinterpret_cast<T*>(smem_raw);  // n x CW
   __shared__ T part[GB][CW];
+  __shared__ T dg[GB][GB + 1];  // the current diagonal block (padded rows)
+  constexpr int DPT = GB * GB / NT;  // diagonal-block entries per thread
 
   // right-hand side slice: B gathered by perm (forward) or X (backward); a
   // thread takes whole rows, 4 rows per pass with their loads in flight
@@ -84,6 +86,14 @@ __global__ __launch_bounds__(NT) void sweep_kernel(int n, int nrhs, const T* LUb
     const int bi = LOWER ? step : nblk - 1 - step;
     const int r0 = bi * GB, bs = min(GB, n - r0);
     const int klo = LOWER ? 0 : r0 + bs, khi = LOWER ? r0 : n;
+    // the diagonal block, loaded by all threads (coalesced, clamped indices)
+    // before the off-diagonal pass so its latency hides behind it
+    T dpre[DPT];
+#pragma unroll
+    for (int u = 0; u < DPT; ++u) {
+      const int e = t + u * NT, rr = e / GB, cc = e - rr * GB;
+      dpre[u] = LU[(int64_t)(r0 + min(rr, bs - 1)) * n + r0 + min(cc, bs - 1)];
+    }
     // ---- off-diagonal contributions: row r of the block; its S threads take
     // interleaved columns (klo + sg + S m), so a row's threads read
     // consecutive factor entries and consecutive rows of the LDS slice
@@ -118,17 +128,15 @@ __global__ __launch_bounds__(NT) void sweep_kernel(int n, int nrhs, const T* LUb
 #pragma unroll
       for (int c = 0; c < CW; ++c) part[r][c] = acc[c];
     }
+#pragma unroll
+    for (int u = 0; u < DPT; ++u) {
+      const int e = t + u * NT, rr = e / GB, cc = e - rr * GB;
+      dg[rr][cc] = (rr < bs && cc < bs) ? dpre[u] : T(0);
+    }
     __syncthreads();
     // ---- diagonal block: wave w solves columns w, w + NWV, ... (lane = row)
     if (wave < CW) {
       const int i = r0 + lane;
-      T row[GB];
-      const T* lr = LU + (int64_t)(lane < bs ? i : r0) * n + r0;
-      // clamped unconditional loads (no per-element branches), then masked
-#pragma unroll
-      for (int q = 0; q < GB; ++q) row[q] = lr[min(q, bs - 1)];
-#pragma unroll
-      for (int q = 0; q < GB; ++q) row[q] = (lane < bs && q < bs) ? row[q] : T(0);
       T x[CPW];
 #pragma unroll
       for (int j = 0; j < CPW; ++j) {
@@ -140,14 +148,12 @@ __global__ __launch_bounds__(NT) void sweep_kernel(int n, int nrhs, const T* LUb
 #pragma unroll
         for (int q = 0; q < GB; ++q) {
           if (q + 1 >= bs) break;
-          const T f = lane > q ? row[q] : T(0);
+          const T f = lane > q ? dg[lane][q] : T(0);
 #pragma unroll
           for (int j = 0; j < CPW; ++j) x[j] = x[j] - f * readlane<T>(x[j], q);
         }
       } else {
-        T d = T(1);
-#pragma unroll
-        for (int q = 0; q < GB; ++q) d = (q == lane && lane < bs) ? row[q] : d;
+        const T d = lane < bs ? dg[lane][lane] : T(1);
         // one division per lane, in parallel; the row's scaling at its step is
         // then a multiply (within one rounding of lu_solve's division)
         const T dinv = T(1) / d;
@@ -158,7 +164,7 @@ __global__ __launch_bounds__(NT) void sweep_kernel(int n, int nrhs, const T* LUb
 #pragma unroll
             for (int j = 0; j < CPW; ++j) x[j] = x[j] * dinv;
           }
-          const T f = lane < q ? row[q] : T(0);
+          const T f = lane < q ? dg[lane][q] : T(0);
 #pragma unroll
           for (int j = 0; j < CPW; ++j) x[j] = x[j] - f * readlane<T>(x[j], q);
         }
@@ -202,17 +208,17 @@ __global__ __launch_bounds__(64) void ipiv_to_perm_kernel(int n, const int32_t* 
 
 }  // namespace
 
-// Dynamic LDS a sweep may use: the device's per-workgroup maximum less room
-// for the static arrays, at most 128 KiB, at least the 64 KiB every launch may
-// use without opting in.
+// Dynamic LDS a sweep may use: the device's per-workgroup maximum less 48 KiB
+// for the static arrays (the diagonal block, up to 64 x 65 doubles, and the
+// partials), at most 128 KiB.
 size_t lds_budget() {
   static const size_t b = [] {
     int dev = 0, v = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
-      return (size_t)64 * 1024;
-    const size_t avail = v > 8 * 1024 ? (size_t)v - 8 * 1024 : 0;
-    return std::max<size_t>(64 * 1024, std::min<size_t>(avail, 128 * 1024));
+      v = 64 * 1024;
+    const size_t avail = v > 64 * 1024 ? (size_t)v - 48 * 1024 : (size_t)16 * 1024;
+    return std::min<size_t>(avail, 128 * 1024);
   }();
   return b;
 }
@@ -242,11 +248,12 @@ int launch_sweep_cw(dim3 grid, size_t shm, hipStream_t s, int n, int nrhs, const
                     const int32_t* perm, const T* B, int64_t strideB, T* X, int64_t strideX, const int32_t* info,
                     const int32_t* mask) {
   auto* k = sweep_kernel<T, LOWER, NT, CW>;
-  if (shm > 64 * 1024) {
-    static const hipError_t once = hipFuncSetAttribute(
-        reinterpret_cast<const void*>(k), hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_budget());
-    NPPH_HIP(once);
-  }
+  // static LDS (diagonal block, partials) plus the slice can pass 64 KiB:
+  // opt in once per instantiation to the full dynamic budget
+  static const hipError_t once = hipFuncSetAttribute(
+      reinterpret_cast<const void*>(k), hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_budget());
+  NPPH_HIP(once);
+  (void)shm;
   hipLaunchKernelGGL(k, grid, dim3(NT), shm, s, n, nrhs, LU, strideA, perm, B, strideB, X, strideX, info, mask);
   return NPPH_OK;
 }
