@@ -61,8 +61,10 @@ struct LeafArgs {
   const int32_t* mask;
 };
 
-template <class T, int R, int IB, bool FULL>
-__global__ __launch_bounds__(PT) void leaf_kernel(LeafArgs<T> a) {
+// NTH threads (512, or 256 for panels of at most 256 rows).
+template <class T, int R, int IB, bool FULL, int NTH>
+__global__ __launch_bounds__(NTH) void leaf_kernel(LeafArgs<T> a) {
+  constexpr int NWL = NTH / 64;
   const int64_t mat = blockIdx.x;
   if (a.mask && a.mask[mat] == 0) return;
   T* __restrict__ A = a.A + mat * a.strideA;
@@ -76,9 +78,9 @@ __global__ __launch_bounds__(PT) void leaf_kernel(LeafArgs<T> a) {
 
   __shared__ T L11[LEAF][LEAF + 1];
   __shared__ T Ublk[LEAF][IB];
-  __shared__ double red_v[2][NW];
-  __shared__ int red_p[2][NW];
-  __shared__ T red_row[2][NW][IB];
+  __shared__ double red_v[2][NWL];
+  __shared__ int red_p[2][NWL];
+  __shared__ T red_row[2][NWL][IB];
   __shared__ int mv_old[2 * IB], mv_new[2 * IB];
   __shared__ int mv_cnt;
   __shared__ T stage[2 * IB][LEAF];
@@ -86,7 +88,7 @@ __global__ __launch_bounds__(PT) void leaf_kernel(LeafArgs<T> a) {
   __shared__ int sing_s;
   extern __shared__ int pid[];  // m entries: leaf-start row of the row now at each position
 
-  for (int i = t; i < m; i += PT) pid[i] = i;
+  for (int i = t; i < m; i += NTH) pid[i] = i;
   if (t == 0) sing_s = a.info[mat];
   __syncthreads();
 
@@ -98,7 +100,7 @@ __global__ __launch_bounds__(PT) void leaf_kernel(LeafArgs<T> a) {
     // ---- gather the block's columns (lazy: rows still at leaf-start places)
 #pragma unroll
     for (int s = 0; s < R; ++s) {
-      const int p0 = jb + t + PT * s;
+      const int p0 = jb + t + NTH * s;
       pos[s] = p0 < m ? p0 : (1 << 28);
       const T* src = A + (int64_t)(k0 + (p0 < m ? pid[p0] : 0)) * n + cb;
 #pragma unroll
@@ -108,17 +110,17 @@ __global__ __launch_bounds__(PT) void leaf_kernel(LeafArgs<T> a) {
     }
     {
       // U rows of the block (loads of every pass issued before any use)
-      constexpr int UIT = ((LEAF - IB) * IB + PT - 1) / PT;
+      constexpr int UIT = ((LEAF - IB) * IB + NTH - 1) / NTH;
       T u[UIT];
 #pragma unroll
       for (int i = 0; i < UIT; ++i) {
-        const int e = t + i * PT, q = e / IB, c = e % IB;
+        const int e = t + i * NTH, q = e / IB, c = e % IB;
         const bool in = e < jb * IB && c < ib;
         u[i] = in ? A[(int64_t)(k0 + pid[in ? q : 0]) * n + cb + c] : T(0);
       }
 #pragma unroll
       for (int i = 0; i < UIT; ++i) {
-        const int e = t + i * PT;
+        const int e = t + i * NTH;
         if (e < jb * IB) Ublk[e / IB][e % IB] = u[i];
       }
     }
@@ -126,7 +128,7 @@ __global__ __launch_bounds__(PT) void leaf_kernel(LeafArgs<T> a) {
     // ---- U rows [0, jb) of the block: wave w solves columns w, w+8, ...
     // (lane = row)
     if (jb > 0) {
-      for (int cc = wave; cc < ib; cc += NW) {
+      for (int cc = wave; cc < ib; cc += NWL) {
         T x = lane < jb ? Ublk[lane][cc] : T(0);
         for (int p = 0; p < jb - 1; ++p) {
           const T xp = readlane<T>(x, p);
@@ -216,7 +218,7 @@ __global__ __launch_bounds__(PT) void leaf_kernel(LeafArgs<T> a) {
       bv = red_v[buf][0];
       bp = red_p[buf][0];
 #pragma unroll
-      for (int q = 1; q < NW; ++q) {
+      for (int q = 1; q < NWL; ++q) {
         const double ov = red_v[buf][q];
         const int op = red_p[buf][q];
         if (piv_better(ov, op, bv, bp)) { bv = ov; bp = op; wbest = q; }
@@ -258,16 +260,16 @@ __global__ __launch_bounds__(PT) void leaf_kernel(LeafArgs<T> a) {
     }
     __syncthreads();
     if (jb > 0) {
-      constexpr int LIT = (IB * (LEAF - IB) + PT - 1) / PT;
+      constexpr int LIT = (IB * (LEAF - IB) + NTH - 1) / NTH;
       T l[LIT];
 #pragma unroll
       for (int i = 0; i < LIT; ++i) {
-        const int e = t + i * PT, rr = e / jb, q = e - rr * jb;
+        const int e = t + i * NTH, rr = e / jb, q = e - rr * jb;
         l[i] = e < ib * jb ? A[(int64_t)(k0 + stage_pid[rr]) * n + k0 + q] : T(0);
       }
 #pragma unroll
       for (int i = 0; i < LIT; ++i) {
-        const int e = t + i * PT, rr = e / jb, q = e - rr * jb;
+        const int e = t + i * NTH, rr = e / jb, q = e - rr * jb;
         if (e < ib * jb) L11[jb + rr][q] = l[i];
       }
     }
@@ -291,21 +293,21 @@ __global__ __launch_bounds__(PT) void leaf_kernel(LeafArgs<T> a) {
     // ---- move the up-to-date left columns, perm and pid of those rows
     const int cnt = mv_cnt;
     const int oc = jb;  // left columns [k0, cb)
-    constexpr int MIT = (2 * IB * (LEAF - IB + 2) + PT - 1) / PT;
+    constexpr int MIT = (2 * IB * (LEAF - IB + 2) + NTH - 1) / NTH;
     const int tot = cnt * (oc + 2);
     if (tot > 0) {
       T mvv[MIT];
       int mvi[MIT];
 #pragma unroll
       for (int i = 0; i < MIT; ++i) {
-        const int e = t + i * PT, q = e / (oc + 2), j = e - q * (oc + 2);
+        const int e = t + i * NTH, q = e / (oc + 2), j = e - q * (oc + 2);
         const int src = e < tot ? mv_old[q] : 0;
         mvv[i] = (e < tot && j < oc) ? A[(int64_t)(k0 + src) * n + k0 + j] : T(0);
         mvi[i] = e < tot ? (j == oc ? perm[k0 + src] : (j > oc ? pid[src] : 0)) : 0;
       }
 #pragma unroll
       for (int i = 0; i < MIT; ++i) {
-        const int e = t + i * PT, q = e / (oc + 2), j = e - q * (oc + 2);
+        const int e = t + i * NTH, q = e / (oc + 2), j = e - q * (oc + 2);
         if (e < tot) {
           if (j < oc) stage[q][j] = mvv[i];
           else if (j == oc) stage_perm[q] = mvi[i];
@@ -315,7 +317,7 @@ __global__ __launch_bounds__(PT) void leaf_kernel(LeafArgs<T> a) {
       __syncthreads();
 #pragma unroll
       for (int i = 0; i < MIT; ++i) {
-        const int e = t + i * PT, q = e / (oc + 2), j = e - q * (oc + 2);
+        const int e = t + i * NTH, q = e / (oc + 2), j = e - q * (oc + 2);
         if (e < tot) {
           const int dst = mv_new[q];
           if (j < oc) A[(int64_t)(k0 + dst) * n + k0 + j] = stage[q][j];
@@ -331,7 +333,7 @@ __global__ __launch_bounds__(PT) void leaf_kernel(LeafArgs<T> a) {
   if (t == 0) { mv_cnt = 0; a.info[mat] = sing_s; }
   __syncthreads();
   int32_t* mv = a.mv + mat * a.mv_stride;
-  for (int i = t; i < m; i += PT) {
+  for (int i = t; i < m; i += NTH) {
     const int src = pid[i];
     if (src != i) {
       const int e = atomicAdd(&mv_cnt, 1);
@@ -484,10 +486,10 @@ struct Rec {
 
   // FULL instantiation when every inner block of the panel is IB wide (the
   // column guards then fold away and the block's stores are unconditional).
-  template <class U, int R, int IB>
-  void launch_leaf(dim3 grid, dim3 block, size_t shm, const LeafArgs<U>& la) {
-    if (la.w % IB == 0) hipLaunchKernelGGL((leaf_kernel<U, R, IB, true>), grid, block, shm, s, la);
-    else hipLaunchKernelGGL((leaf_kernel<U, R, IB, false>), grid, block, shm, s, la);
+  template <class U, int R, int IB, int NTH = PT>
+  void launch_leaf(dim3 grid, size_t shm, const LeafArgs<U>& la) {
+    if (la.w % IB == 0) hipLaunchKernelGGL((leaf_kernel<U, R, IB, true, NTH>), grid, dim3(NTH), shm, s, la);
+    else hipLaunchKernelGGL((leaf_kernel<U, R, IB, false, NTH>), grid, dim3(NTH), shm, s, la);
   }
 
   int leaf(int k0, int w) {
@@ -495,15 +497,18 @@ struct Rec {
                    (int64_t)nleaf * LSTRIDE, mask};
     const int m = n - k0;
     const size_t shm = (size_t)m * sizeof(int);
-    dim3 grid((unsigned)count), block(PT);
+    dim3 grid((unsigned)count);
     ProfScope ps(kProfPanel, (double)count * m * w * w, 2.0 * sizeof(T) * count * m * w, s);
     // inner block width: as wide as the registers allow (fewer block-level
     // global round trips per leaf)
     constexpr bool F = sizeof(T) == 4;
-    if (m <= PT) launch_leaf<T, 1, F ? 32 : 16>(grid, block, shm, la);
-    else if (m <= 2 * PT) launch_leaf<T, 2, F ? 32 : 16>(grid, block, shm, la);
-    else if (m <= 4 * PT) launch_leaf<T, 4, F ? 16 : 8>(grid, block, shm, la);
-    else launch_leaf<T, 8, F ? 8 : 4>(grid, block, shm, la);
+    // short panels: 256 threads (half of a 512-thread workgroup would hold
+    // no rows, and the per-column barrier spans fewer waves)
+    if (m <= PT / 2) launch_leaf<T, 1, F ? 32 : 16, PT / 2>(grid, shm, la);
+    else if (m <= PT) launch_leaf<T, 1, F ? 32 : 16>(grid, shm, la);
+    else if (m <= 2 * PT) launch_leaf<T, 2, F ? 32 : 16>(grid, shm, la);
+    else if (m <= 4 * PT) launch_leaf<T, 4, F ? 16 : 8>(grid, shm, la);
+    else launch_leaf<T, 8, F ? 8 : 4>(grid, shm, la);
     NPPH_HIP(hipGetLastError());
     return NPPH_OK;
   }
