@@ -503,7 +503,8 @@ struct Rec {
     // global round trips per leaf)
     constexpr bool F = sizeof(T) == 4;
     // short panels: 256 threads (half of a 512-thread workgroup would hold
-    // no rows, and the per-column barrier spans fewer waves)
+    // no rows, and the per-column barrier spans fewer waves); taller panels
+    // measured faster with 512 threads and fewer rows per thread
     if (m <= PT / 2) launch_leaf<T, 1, F ? 32 : 16, PT / 2>(grid, shm, la);
     else if (m <= PT) launch_leaf<T, 1, F ? 32 : 16>(grid, shm, la);
     else if (m <= 2 * PT) launch_leaf<T, 2, F ? 32 : 16>(grid, shm, la);
