@@ -414,46 +414,56 @@ __global__ __launch_bounds__(256) void move_rows_kernel(T* Abase, int64_t stride
 }
 
 // B <- L^-1 B for the unit-lower s x s block L at (r0, r0), s <= 64, and the
-// columns [c0, c0+nc). One thread per column, L staged in LDS.
+// columns [c0, c0+nc). One thread per column (TT threads share one copy of L,
+// stored transposed in LDS), column-oriented substitution: step p subtracts
+// L[q][p] x[p] from every later row q, so the dependent chain is one FMA per
+// step rather than a q-term dot product per row.
+constexpr int TT = 256;
 template <class T>
-__global__ __launch_bounds__(64) void trsm_leaf_kernel(T* Abase, int64_t strideA, int n, int r0, int s, int c0,
+__global__ __launch_bounds__(TT) void trsm_leaf_kernel(T* Abase, int64_t strideA, int n, int r0, int s, int c0,
                                                        int nc, const int32_t* mask) {
   const int64_t mat = blockIdx.y;
   if (mask && mask[mat] == 0) return;
   T* A = Abase + mat * strideA;
-  __shared__ T L[LEAF][LEAF + 1];
+  __shared__ T LT[LEAF][LEAF + 1];  // LT[p][q] = L[q][p]
   {
-    // thread j loads column j of L: all 64 loads in flight before the LDS writes
-    const int j = threadIdx.x;
-    T colv[LEAF];
+    // coalesced clamped loads of L (row i, column j), transposed into LDS
+    constexpr int PER = LEAF * LEAF / TT;
+    T v[PER];
 #pragma unroll
-    for (int i = 0; i < LEAF; ++i) colv[i] = (i < s && j < s) ? A[(int64_t)(r0 + i) * n + r0 + j] : T(0);
+    for (int u = 0; u < PER; ++u) {
+      const int e = threadIdx.x + u * TT, i = e / LEAF, j = e - i * LEAF;
+      v[u] = A[(int64_t)(r0 + min(i, s - 1)) * n + r0 + min(j, s - 1)];
+    }
 #pragma unroll
-    for (int i = 0; i < LEAF; ++i) L[i][j] = colv[i];
+    for (int u = 0; u < PER; ++u) {
+      const int e = threadIdx.x + u * TT, i = e / LEAF, j = e - i * LEAF;
+      LT[j][i] = (i < s && j < s) ? v[u] : T(0);
+    }
   }
   __syncthreads();
-  const int c = blockIdx.x * 64 + threadIdx.x;
+  const int c = blockIdx.x * TT + threadIdx.x;
   if (c >= nc) return;
-  T* base = A + (int64_t)r0 * n + c0;  // row q of the block at base + q * n
+  T* base = A + (int64_t)r0 * n + c0 + c;  // row q of this column at base + q * n
   T x[LEAF];
 #pragma unroll
-  for (int q = 0; q < LEAF; ++q) x[q] = q < s ? base[(int64_t)q * n + c] : T(0);
+  for (int q = 0; q < LEAF; ++q) x[q] = base[(int64_t)min(q, s - 1) * n];
 #pragma unroll
-  for (int q = 1; q < LEAF; ++q) {
-    // An opaque zero that depends on row q-2's result: row q's L reads cannot
-    // be hoisted above it, so only about two rows of L are live at a time
-    // (hoisting all 2016 LDS reads otherwise spills the whole kernel).
+  for (int q = 0; q < LEAF; ++q) x[q] = q < s ? x[q] : T(0);
+#pragma unroll
+  for (int p = 0; p < LEAF - 1; ++p) {
+    // An opaque zero that depends on a recent result: step p's L reads cannot
+    // be hoisted far above it (hoisting all 2016 LDS reads spills the kernel).
     int dep = 0;
-    if (q >= 2) asm volatile("v_mov_b32 %0, 0" : "=v"(dep) : "v"(x[q - 2]));
-    const T* Lq = &L[q][0] + dep;
-    T acc = x[q];
+    if (p >= 1) asm volatile("v_mov_b32 %0, 0" : "=v"(dep) : "v"(x[sizeof(T) == 4 ? p - 1 : p]));
+    const T* lp = &LT[p][0] + dep;
+    const T xp = x[p];
 #pragma unroll
-    for (int p = 0; p < q; ++p) acc = acc - Lq[p] * x[p];
-    x[q] = acc;
+    for (int q = p + 1; q < LEAF; ++q) x[q] = x[q] - lp[q] * xp;
   }
 #pragma unroll
   for (int q = 0; q < LEAF; ++q)
-    if (q < s) base[(int64_t)q * n + c] = x[q];
+    if (q < s) base[(int64_t)q * n] = x[q];
 }
 
 __global__ void init_factor_kernel(int64_t count, int n, int32_t* perm, int32_t* info) {
@@ -549,9 +559,9 @@ struct Rec {
   int trsm(int r0, int sz, int c0, int nc) {
     if (nc <= 0) return NPPH_OK;
     if (sz <= LEAF) {
-      dim3 grid((unsigned)ceil_div(nc, 64), (unsigned)count);
+      dim3 grid((unsigned)ceil_div(nc, TT), (unsigned)count);
       ProfScope ps(kProfMoveTrsm, (double)count * sz * sz * nc, 2.0 * sizeof(T) * count * (double)sz * nc, s);
-      hipLaunchKernelGGL(trsm_leaf_kernel<T>, grid, dim3(64), 0, s, A, strideA, n, r0, sz, c0, nc, mask);
+      hipLaunchKernelGGL(trsm_leaf_kernel<T>, grid, dim3(TT), 0, s, A, strideA, n, r0, sz, c0, nc, mask);
       NPPH_HIP(hipGetLastError());
       return NPPH_OK;
     }
