@@ -400,17 +400,60 @@ __global__ __launch_bounds__(256) void move_rows_kernel(T* Abase, int64_t stride
   T buf[16];
   if (phase == 0) {
 #pragma unroll
-    for (int q = 0; q < 16; ++q) buf[q] = e0 + q < e1 ? A[(int64_t)mv[1 + 2 * (e0 + q)] * n + c0 + c] : T(0);
+    for (int q = 0; q < 16; ++q) buf[q] = A[(int64_t)mv[1 + 2 * min(e0 + q, e1 - 1)] * n + c0 + c];
 #pragma unroll
     for (int q = 0; q < 16; ++q)
       if (e0 + q < e1) sc[(int64_t)(e0 + q) * nc + c] = buf[q];
   } else {
 #pragma unroll
-    for (int q = 0; q < 16; ++q) buf[q] = e0 + q < e1 ? sc[(int64_t)(e0 + q) * nc + c] : T(0);
+    for (int q = 0; q < 16; ++q) buf[q] = sc[(int64_t)min(e0 + q, e1 - 1) * nc + c];
 #pragma unroll
     for (int q = 0; q < 16; ++q)
       if (e0 + q < e1) A[(int64_t)mv[2 + 2 * (e0 + q)] * n + c0 + c] = buf[q];
   }
+}
+
+// Row moves for columns [c0, c0+nc) in one pass when at most FMV rows move:
+// a workgroup owns 16 columns, stages every moved row's values for them in
+// LDS (the move list first, so the value loads are independent), then writes
+// them to their destinations. Columns are disjoint between workgroups, so no
+// other workgroup reads what this one overwrites.
+constexpr int FMC = 16;
+template <class T>
+constexpr int fused_max_moves() { return sizeof(T) == 4 ? 512 : 256; }
+template <class T>
+__global__ __launch_bounds__(256) void move_rows_fused_kernel(T* Abase, int64_t strideA, int n, int c0, int nc,
+                                                              const int32_t* list, int64_t list_stride,
+                                                              const int32_t* mask) {
+  constexpr int FMV = fused_max_moves<T>();
+  const int64_t mat = blockIdx.y;
+  if (mask && mask[mat] == 0) return;
+  const int32_t* mv = list + mat * list_stride;
+  const int cnt = mv[0];
+  if (cnt == 0) return;
+  T* A = Abase + mat * strideA;
+  __shared__ T buf[FMV][FMC];
+  __shared__ int src[FMV], dst[FMV];
+  const int t = threadIdx.x, cl = t % FMC, g = t / FMC;
+  for (int e = t; e < cnt; e += 256) {
+    src[e] = mv[1 + 2 * e];
+    dst[e] = mv[2 + 2 * e];
+  }
+  __syncthreads();
+  const int c = blockIdx.x * FMC + cl;
+  const int cc = c0 + min(c, nc - 1);
+  constexpr int G = 256 / FMC;  // move groups
+  for (int e0 = g; e0 < cnt; e0 += 4 * G) {
+    T v[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) v[u] = A[(int64_t)src[min(e0 + u * G, cnt - 1)] * n + cc];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (e0 + u * G < cnt) buf[e0 + u * G][cl] = v[u];
+  }
+  __syncthreads();
+  if (c < nc)
+    for (int e = g; e < cnt; e += G) A[(int64_t)dst[e] * n + c0 + c] = buf[e][cl];
 }
 
 // B <- L^-1 B for the unit-lower s x s block L at (r0, r0), s <= 64, and the
@@ -533,9 +576,15 @@ struct Rec {
                        nleaf, l0, l1, net, (int64_t)(1 + 2 * n), mask);
     NPPH_HIP(hipGetLastError());
     const int maxmv = std::min(m, 2 * wl);
+    ProfScope ps(kProfMoveTrsm, 0.0, 4.0 * sizeof(T) * count * (double)maxmv * nc, s);
+    if (maxmv <= fused_max_moves<T>()) {
+      hipLaunchKernelGGL(move_rows_fused_kernel<T>, dim3((unsigned)ceil_div(nc, FMC), (unsigned)count), dim3(256), 0,
+                         s, A, strideA, n, c0, nc, net, (int64_t)(1 + 2 * n), mask);
+      NPPH_HIP(hipGetLastError());
+      return NPPH_OK;
+    }
     if ((int64_t)maxmv * nc > scr_stride) { set_error("LU: row-move scratch too small"); return NPPH_ERR_INVALID; }
     dim3 grid((unsigned)ceil_div(nc, 256), (unsigned)ceil_div(maxmv, 16), (unsigned)count);
-    ProfScope ps(kProfMoveTrsm, 0.0, 4.0 * sizeof(T) * count * (double)maxmv * nc, s);
     for (int phase = 0; phase < 2; ++phase) {
       hipLaunchKernelGGL(move_rows_kernel<T>, grid, dim3(256), 0, s, A, strideA, n, c0, nc, net, (int64_t)(1 + 2 * n),
                          scratch, scr_stride, phase, mask);
