@@ -105,6 +105,9 @@ __device__ __forceinline__ float readlane<float>(float v, int lane) { return rea
 __device__ __forceinline__ bool piv_better(double v1, int p1, double v0, int p0) {
   return v1 > v0 || (v1 == v0 && p1 < p0);
 }
+__device__ __forceinline__ bool piv_better(float v1, int p1, float v0, int p0) {
+  return v1 > v0 || (v1 == v0 && p1 < p0);
+}
 
 // One exchange step of the argmax: each lane compares with the lane that the
 // DPP pattern CTRL names (all patterns used stay inside a 16-lane row).
@@ -119,6 +122,14 @@ __device__ __forceinline__ void argmax_dpp(double& v, int& pos) {
   if (piv_better(ov, op, v, pos)) { v = ov; pos = op; }
 }
 
+template <int CTRL>
+__device__ __forceinline__ void argmax_dpp(float& v, int& pos) {
+  const int o = __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), CTRL, 0xF, 0xF, false);
+  const int op = __builtin_amdgcn_mov_dpp(pos, CTRL, 0xF, 0xF, false);
+  const float ov = __builtin_bit_cast(float, o);
+  if (piv_better(ov, op, v, pos)) { v = ov; pos = op; }
+}
+
 // Wave-wide argmax of (v, pos); every lane receives the winner. Within each
 // 16-lane row: DPP swaps with the neighbouring lane, the neighbouring pair,
 // the mirrored half-row and the mirrored row (quad_perm [1,0,3,2], quad_perm
@@ -126,16 +137,17 @@ __device__ __forceinline__ void argmax_dpp(double& v, int& pos) {
 // its lanes; the four row winners are then read with v_readlane. The order is
 // total (largest value, then smallest position), so the result does not
 // depend on the reduction tree.
-__device__ __forceinline__ void wave_argmax(double& v, int& pos) {
+template <class V>
+__device__ __forceinline__ void wave_argmax(V& v, int& pos) {
   argmax_dpp<0xB1>(v, pos);
   argmax_dpp<0x4E>(v, pos);
   argmax_dpp<0x141>(v, pos);
   argmax_dpp<0x140>(v, pos);
-  double bv = readlane_d(v, 0);
+  V bv = readlane<V>(v, 0);
   int bp = __builtin_amdgcn_readlane(pos, 0);
 #pragma unroll
   for (int r = 1; r < 4; ++r) {
-    const double ov = readlane_d(v, 16 * r);
+    const V ov = readlane<V>(v, 16 * r);
     const int op = __builtin_amdgcn_readlane(pos, 16 * r);
     if (piv_better(ov, op, bv, bp)) { bv = ov; bp = op; }
   }

@@ -78,7 +78,7 @@ __global__ __launch_bounds__(NTH) void leaf_kernel(LeafArgs<T> a) {
 
   __shared__ T L11[LEAF][LEAF + 1];
   __shared__ T Ublk[LEAF][IB];
-  __shared__ double red_v[2][NWL];
+  __shared__ T red_v[2][NWL];  // candidate magnitudes (exact in T)
   __shared__ int red_p[2][NWL];
   __shared__ T red_row[2][NWL][IB];
   __shared__ int mv_old[2 * IB], mv_new[2 * IB];
@@ -187,16 +187,16 @@ __global__ __launch_bounds__(NTH) void leaf_kernel(LeafArgs<T> a) {
       if (c >= ib) return;
       const int col = jb + c;  // current leaf-relative step
       const int buf = c & 1;
-      double bv = -1.0;
+      T bv = T(-1);
       int bp = 1 << 28, bs = -1;
 #pragma unroll
       for (int s = 0; s < R; ++s) {
         if (pos[s] >= col && pos[s] < m) {
-          const double av = fabs((double)v[s][c]);
+          const T av = fabs(v[s][c]);
           if (piv_better(av, pos[s], bv, bp)) { bv = av; bp = pos[s]; bs = s; }
         }
       }
-      double wv = bv;
+      T wv = bv;
       int wp = bp;
       wave_argmax(wv, wp);
       if (bs >= 0 && wp == bp) {
@@ -209,8 +209,8 @@ __global__ __launch_bounds__(NTH) void leaf_kernel(LeafArgs<T> a) {
 #pragma unroll
             for (int j = c; j < IB; ++j) red_row[buf][wave][j] = v[s][j];
           }
-      } else if (lane == 0 && wv < 0.0) {
-        red_v[buf][wave] = -1.0;
+      } else if (lane == 0 && wv < T(0)) {
+        red_v[buf][wave] = T(-1);
         red_p[buf][wave] = 1 << 28;
       }
       __syncthreads();
@@ -219,14 +219,17 @@ __global__ __launch_bounds__(NTH) void leaf_kernel(LeafArgs<T> a) {
       bp = red_p[buf][0];
 #pragma unroll
       for (int q = 1; q < NWL; ++q) {
-        const double ov = red_v[buf][q];
+        const T ov = red_v[buf][q];
         const int op = red_p[buf][q];
         if (piv_better(ov, op, bv, bp)) { bv = ov; bp = op; wbest = q; }
       }
       const int p = bp;
-      if (bv == 0.0 && t == 0 && sing_s == 0) sing_s = k0 + col + 1;
+      if (bv == T(0) && t == 0 && sing_s == 0) sing_s = k0 + col + 1;
       if (t == 0) ipiv[k0 + col] = k0 + p;
       const T piv = red_row[buf][wbest][c];
+      // FP32 factors (refined afterwards) scale by one reciprocal per column;
+      // FP64 factors divide, as linalg.cpp:65
+      const T rpiv = sizeof(T) == 4 && piv != T(0) ? T(1) / piv : T(0);
 #pragma unroll
       for (int s = 0; s < R; ++s) {
         if (pos[s] == p) pos[s] = col;
@@ -235,7 +238,7 @@ __global__ __launch_bounds__(NTH) void leaf_kernel(LeafArgs<T> a) {
 #pragma unroll
       for (int s = 0; s < R; ++s) {
         if (pos[s] > col && pos[s] < m) {
-          const T l = piv != T(0) ? v[s][c] / piv : T(0);
+          const T l = sizeof(T) == 4 ? v[s][c] * rpiv : (piv != T(0) ? v[s][c] / piv : T(0));
           v[s][c] = l;
 #pragma unroll
           for (int j = c + 1; j < IB; ++j) v[s][j] = v[s][j] - l * red_row[buf][wbest][j];
