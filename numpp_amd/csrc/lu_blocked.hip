@@ -76,14 +76,16 @@ __global__ __launch_bounds__(NTH) void leaf_kernel(LeafArgs<T> a) {
   constexpr int QC = (R >= 4 && sizeof(T) == 8) ? 4 : 8;  // L columns per update chunk
   static_assert(IB % QC == 0 && LEAF % IB == 0, "inner blocks must hold whole update chunks");
 
+  // rows/columns left of a block: at most LEAF - IB (none when IB = LEAF)
+  constexpr int LEFT = LEAF - IB > 0 ? LEAF - IB : 1;
   __shared__ T L11[LEAF][LEAF + 1];
-  __shared__ T Ublk[LEAF][IB];
+  __shared__ T Ublk[LEFT][IB];
   __shared__ T red_v[2][NWL];  // candidate magnitudes (exact in T)
   __shared__ int red_p[2][NWL];
   __shared__ T red_row[2][NWL][IB];
   __shared__ int mv_old[2 * IB], mv_new[2 * IB];
   __shared__ int mv_cnt;
-  __shared__ T stage[2 * IB][LEAF];
+  __shared__ T stage[2 * IB][LEFT];
   __shared__ int stage_perm[2 * IB], stage_pid[2 * IB];
   __shared__ int sing_s;
   extern __shared__ int pid[];  // m entries: leaf-start row of the row now at each position
@@ -110,7 +112,7 @@ __global__ __launch_bounds__(NTH) void leaf_kernel(LeafArgs<T> a) {
     }
     {
       // U rows of the block (loads of every pass issued before any use)
-      constexpr int UIT = ((LEAF - IB) * IB + NTH - 1) / NTH;
+      constexpr int UIT = ((LEAF - IB) * IB + NTH - 1) / NTH > 0 ? ((LEAF - IB) * IB + NTH - 1) / NTH : 1;
       T u[UIT];
 #pragma unroll
       for (int i = 0; i < UIT; ++i) {
@@ -263,7 +265,7 @@ __global__ __launch_bounds__(NTH) void leaf_kernel(LeafArgs<T> a) {
     }
     __syncthreads();
     if (jb > 0) {
-      constexpr int LIT = (IB * (LEAF - IB) + NTH - 1) / NTH;
+      constexpr int LIT = (IB * (LEAF - IB) + NTH - 1) / NTH > 0 ? (IB * (LEAF - IB) + NTH - 1) / NTH : 1;
       T l[LIT];
 #pragma unroll
       for (int i = 0; i < LIT; ++i) {
@@ -561,9 +563,9 @@ struct Rec {
     // short panels: 256 threads (half of a 512-thread workgroup would hold
     // no rows, and the per-column barrier spans fewer waves); taller panels
     // measured faster with 512 threads and fewer rows per thread
-    if (m <= PT / 2) launch_leaf<T, 1, F ? 32 : 16, PT / 2>(grid, shm, la);
-    else if (m <= PT) launch_leaf<T, 1, F ? 32 : 16>(grid, shm, la);
-    else if (m <= 2 * PT) launch_leaf<T, 2, F ? 32 : 16>(grid, shm, la);
+    if (m <= PT / 2) launch_leaf<T, 1, 64, PT / 2>(grid, shm, la);
+    else if (m <= PT) launch_leaf<T, 1, 64>(grid, shm, la);
+    else if (m <= 2 * PT) launch_leaf<T, 2, F ? 64 : 16>(grid, shm, la);
     else if (m <= 4 * PT) launch_leaf<T, 4, F ? 16 : 8>(grid, shm, la);
     else launch_leaf<T, 8, F ? 8 : 4>(grid, shm, la);
     NPPH_HIP(hipGetLastError());
