@@ -566,7 +566,7 @@ struct Rec {
     if (m <= PT / 2) launch_leaf<T, 1, 64, PT / 2>(grid, shm, la);
     else if (m <= PT) launch_leaf<T, 1, 64>(grid, shm, la);
     else if (m <= 2 * PT) launch_leaf<T, 2, F ? 64 : 16>(grid, shm, la);
-    else if (m <= 4 * PT) launch_leaf<T, 4, F ? 16 : 8>(grid, shm, la);
+    else if (m <= 4 * PT) launch_leaf<T, 4, F ? 32 : 8>(grid, shm, la);
     else launch_leaf<T, 8, F ? 8 : 4>(grid, shm, la);
     NPPH_HIP(hipGetLastError());
     return NPPH_OK;
