@@ -25,7 +25,7 @@ namespace npph {
 
 namespace {
 
-constexpr int BM = 128, BK = 16, NT = 256;
+constexpr int BK = 16, NT = 256;
 constexpr int PAD = 4;
 
 template <class T>
@@ -61,11 +61,15 @@ __device__ __forceinline__ void tile_coords(int bid, int tiles_m, int tiles_n, i
   tn = r / gsize;
 }
 
-// BN = 128 (wave tile 64 x 64) or 64 (wave tile 64 x 32, for narrow updates).
-template <class T, int BN>
+// Tile BM x BN with BM, BN in {128, 64}: the waves form a 2 x 2 grid of
+// (BM/2) x (BN/2) sub-tiles (64-wide tiles serve narrow or short products, where
+// a 128-wide tile would leave MFMA rows or columns empty).
+template <class T, int BM, int BN>
 __global__ __launch_bounds__(NT, 2) void gemm_kernel(GemmArgs<T> p) {
   typedef typename Mfma<T>::acc_t acc_t;
+  constexpr int IT = BM / 32;      // 16-row MFMA tiles per wave
   constexpr int JT = BN / 32;      // 16-column MFMA tiles per wave
+  constexpr int AE = BM / 16;      // A elements per thread per slice
   constexpr int BE = BN / 16;      // B elements per thread per slice
   __shared__ T As[2][BK][BM + PAD];
   __shared__ T Bs[2][BK][BN + PAD];
@@ -81,12 +85,12 @@ __global__ __launch_bounds__(NT, 2) void gemm_kernel(GemmArgs<T> p) {
   const int lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
 
-  // Global-load mapping: 8 elements per thread per operand.
-  const int a_row = tid >> 1, a_k = (tid & 1) * 8;    // A tile: BM rows x BK
+  // Global-load mapping: AE (BE) consecutive elements per thread and operand.
+  const int a_row = tid / (BK / AE), a_k = (tid % (BK / AE)) * AE;  // A tile: BM rows x BK
   const int b_k = tid >> 4, b_col = (tid & 15) * BE;  // B tile: BK rows x BN
   const bool mn_full = (m0 + BM <= p.M) && (n0 + BN <= p.N);
 
-  T ra[8], rb[BE];
+  T ra[AE], rb[BE];
   auto load = [&](int64_t k0) {
     const bool full = mn_full && (k0 + BK <= p.K) && p.vec;
     const int64_t gr = m0 + a_row, gk = k0 + a_k;
@@ -96,7 +100,7 @@ __global__ __launch_bounds__(NT, 2) void gemm_kernel(GemmArgs<T> p) {
       const T* pb = B + bk * p.ldb + bc;
       if constexpr (sizeof(T) == 8) {
 #pragma unroll
-        for (int i = 0; i < 8; i += 2) {
+        for (int i = 0; i < AE; i += 2) {
           double2 va = *reinterpret_cast<const double2*>(pa + i);
           ra[i] = va.x; ra[i + 1] = va.y;
         }
@@ -107,7 +111,7 @@ __global__ __launch_bounds__(NT, 2) void gemm_kernel(GemmArgs<T> p) {
         }
       } else {
 #pragma unroll
-        for (int i = 0; i < 8; i += 4) {
+        for (int i = 0; i < AE; i += 4) {
           float4 va = *reinterpret_cast<const float4*>(pa + i);
           ra[i] = va.x; ra[i + 1] = va.y; ra[i + 2] = va.z; ra[i + 3] = va.w;
         }
@@ -119,21 +123,21 @@ __global__ __launch_bounds__(NT, 2) void gemm_kernel(GemmArgs<T> p) {
       }
     } else {
 #pragma unroll
-      for (int i = 0; i < 8; ++i) ra[i] = (gr < p.M && gk + i < p.K) ? A[gr * p.lda + gk + i] : T(0);
+      for (int i = 0; i < AE; ++i) ra[i] = (gr < p.M && gk + i < p.K) ? A[gr * p.lda + gk + i] : T(0);
 #pragma unroll
       for (int i = 0; i < BE; ++i) rb[i] = (bk < p.K && bc + i < p.N) ? B[bk * p.ldb + bc + i] : T(0);
     }
   };
   auto store = [&](int buf) {
 #pragma unroll
-    for (int i = 0; i < 8; ++i) As[buf][a_k + i][a_row] = ra[i];
+    for (int i = 0; i < AE; ++i) As[buf][a_k + i][a_row] = ra[i];
 #pragma unroll
     for (int i = 0; i < BE; ++i) Bs[buf][b_k][b_col + i] = rb[i];
   };
 
-  acc_t acc[4][JT];
+  acc_t acc[IT][JT];
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < IT; ++i)
 #pragma unroll
     for (int j = 0; j < JT; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
 
@@ -147,13 +151,13 @@ __global__ __launch_bounds__(NT, 2) void gemm_kernel(GemmArgs<T> p) {
     if (kt + 1 < kt_total) load((int64_t)(kt + 1) * BK);
 #pragma unroll
     for (int kk = 0; kk < BK; kk += 4) {
-      T af[4], bf[JT];
+      T af[IT], bf[JT];
 #pragma unroll
-      for (int i = 0; i < 4; ++i) af[i] = As[cur][kk + lk][wm * 64 + i * 16 + lr];
+      for (int i = 0; i < IT; ++i) af[i] = As[cur][kk + lk][wm * (BM / 2) + i * 16 + lr];
 #pragma unroll
       for (int j = 0; j < JT; ++j) bf[j] = Bs[cur][kk + lk][wn * (BN / 2) + j * 16 + lr];
 #pragma unroll
-      for (int i = 0; i < 4; ++i)
+      for (int i = 0; i < IT; ++i)
 #pragma unroll
         for (int j = 0; j < JT; ++j) acc[i][j] = Mfma<T>::run(af[i], bf[j], acc[i][j]);
     }
@@ -164,10 +168,10 @@ __global__ __launch_bounds__(NT, 2) void gemm_kernel(GemmArgs<T> p) {
   // Epilogue: C[row][col] = alpha * acc + beta * C.
   const bool beta0 = (p.beta == T(0));
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {
+  for (int i = 0; i < IT; ++i) {
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
-      const int64_t row = m0 + wm * 64 + i * 16 + Mfma<T>::row(lk, v);
+      const int64_t row = m0 + wm * (BM / 2) + i * 16 + Mfma<T>::row(lk, v);
       if (row >= p.M) continue;
       T* crow = C + row * p.ldc;
 #pragma unroll
@@ -205,10 +209,11 @@ int gemm_launch(GemmArgs<T> p, int64_t batch, hipStream_t s, int prof) {
     set_error("gemm: leading dimension too small");
     return NPPH_ERR_INVALID;
   }
-  // 64-wide column tiles for narrow products (the lowest recursion levels),
-  // where a 128-wide tile would leave half of its MFMA columns empty
-  const bool narrow = p.N <= 64;
-  const int BN = narrow ? 64 : 128;
+  // 64-wide tiles for narrow (N <= 64) or short (M <= 64) products (the
+  // lowest recursion levels), where a 128-wide tile would leave half of its
+  // MFMA columns or rows empty
+  const bool narrow = p.N <= 64, shortm = p.M <= 64;
+  const int BN = narrow ? 64 : 128, BM = shortm ? 64 : 128;
   p.tiles_m = (int)ceil_div(p.M, BM);
   p.tiles_n = (int)ceil_div(p.N, BN);
   p.vec = vec_ok(p) ? 1 : 0;
@@ -218,8 +223,10 @@ int gemm_launch(GemmArgs<T> p, int64_t batch, hipStream_t s, int prof) {
   const double by = es * batch * ((double)p.M * p.K + (double)p.K * p.N + (p.beta == T(0) ? 1.0 : 2.0) * p.M * p.N);
   const int kind = prof == -1 ? (sizeof(T) == 8 ? kProfGemmF64 : kProfGemmF32) : prof;
   ProfScope ps(kind, fl, by, s);
-  if (narrow) hipLaunchKernelGGL((gemm_kernel<T, 64>), grid, dim3(NT), 0, s, p);
-  else hipLaunchKernelGGL((gemm_kernel<T, 128>), grid, dim3(NT), 0, s, p);
+  if (shortm && narrow) hipLaunchKernelGGL((gemm_kernel<T, 64, 64>), grid, dim3(NT), 0, s, p);
+  else if (shortm) hipLaunchKernelGGL((gemm_kernel<T, 64, 128>), grid, dim3(NT), 0, s, p);
+  else if (narrow) hipLaunchKernelGGL((gemm_kernel<T, 128, 64>), grid, dim3(NT), 0, s, p);
+  else hipLaunchKernelGGL((gemm_kernel<T, 128, 128>), grid, dim3(NT), 0, s, p);
   NPPH_HIP(hipGetLastError());
   return NPPH_OK;
 }
