@@ -35,7 +35,8 @@ __global__ __launch_bounds__(NT) void sweep_kernel(int n, int nrhs, const T* LUb
   constexpr int NWV = NT / 64;     // waves
   constexpr int CPW = (CW + NWV - 1) / NWV;  // columns per wave in the diagonal solve
   // factor entries per thread and step: deep for long rows (n > 256), short
-  // for small orders, whose ranges are at most 192 columns
+  // for small orders, where more resident workgroups hide the latency better
+  // than deeper batches (measured)
   constexpr int M = NT > 256 ? (sizeof(T) == 8 ? 16 : 32) : 8;
   const int64_t mat = blockIdx.x;
   if ((info && info[mat] != 0) || (mask && mask[mat] == 0)) return;
@@ -50,26 +51,27 @@ __global__ __launch_bounds__(NT) void sweep_kernel(int n, int nrhs, const T* LUb
   constexpr int DPT = GB * GB / NT;  // diagonal-block entries per thread
 
   // right-hand side slice: B gathered by perm (forward) or X (backward); a
-  // thread takes whole rows, 4 rows per pass with their loads in flight
+  // thread takes whole rows, RP rows per pass with their loads in flight
   {
+    constexpr int RP = CW >= 16 ? 2 : 4;
     const T* Bm = Bb + mat * strideB;
     const int32_t* perm = permb + mat * n;
-    for (int i0 = t; i0 < n; i0 += 4 * NT) {
-      int src[4];
+    for (int i0 = t; i0 < n; i0 += RP * NT) {
+      int src[RP];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
+      for (int u = 0; u < RP; ++u) {
         const int i = min(i0 + u * NT, n - 1);
         src[u] = LOWER ? perm[i] : i;
       }
-      T v[4][CW];
+      T v[RP][CW];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
+      for (int u = 0; u < RP; ++u) {
         const T* br = (LOWER ? Bm : X) + (int64_t)src[u] * nrhs + c0;
 #pragma unroll
         for (int c = 0; c < CW; ++c) v[u][c] = c < nc ? br[c] : T(0);
       }
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
+      for (int u = 0; u < RP; ++u) {
         const int i = i0 + u * NT;
         if (i < n) {
 #pragma unroll
@@ -223,21 +225,24 @@ size_t lds_budget() {
   return b;
 }
 
-// Right-hand-side columns per workgroup: the widest of 8, 4, 2, 1 whose n x CW
-// slice fits in LDS and that still gives at least one workgroup per CU (each
-// workgroup streams its matrix's whole triangle, so few wide workgroups leave
-// most CUs idle). NUMPP_HIP_GETRS_CW = 1, 2, 4 or 8 caps it.
+// Right-hand-side columns per workgroup: the widest of 16, 8, 4, 2, 1 whose
+// n x CW slice fits in LDS, that still gives at least one workgroup per CU
+// (each workgroup streams its matrix's whole triangle, so few wide workgroups
+// leave most CUs idle) and that is less than twice nrhs.
+// NUMPP_HIP_GETRS_CW = 1, 2, 4 or 8 caps it.
 template <class T>
 int chunk_width(int64_t count, int n, int nrhs) {
   static const int cap_env = [] {
     const char* e = std::getenv("NUMPP_HIP_GETRS_CW");
-    const int v = e ? std::atoi(e) : 8;
-    return (v == 1 || v == 2 || v == 4) ? v : 8;
+    const int v = e ? std::atoi(e) : 16;
+    return (v == 1 || v == 2 || v == 4 || v == 8) ? v : 16;
   }();
   const size_t cap = lds_budget();
   const int64_t want = cu_count() > 0 ? cu_count() : 256;
-  int cw = cap_env;
-  while (cw > 1 && ((size_t)n * cw * sizeof(T) > cap || count * ceil_div(nrhs, cw) < want)) cw /= 2;
+  // FP64 slices of 16 columns hold too much LDS for enough workgroups per CU
+  int cw = sizeof(T) == 8 ? std::min(cap_env, 8) : cap_env;
+  while (cw > 1 && ((size_t)n * cw * sizeof(T) > cap || count * ceil_div(nrhs, cw) < want || cw >= 2 * nrhs))
+    cw /= 2;
   return cw;
 }
 
@@ -262,6 +267,11 @@ template <class T, bool LOWER, int NT>
 int launch_sweep(int cw, dim3 grid, size_t shm, hipStream_t s, int n, int nrhs, const T* LU, int64_t strideA,
                  const int32_t* perm, const T* B, int64_t strideB, T* X, int64_t strideX, const int32_t* info,
                  const int32_t* mask) {
+  if constexpr (sizeof(T) == 4) {
+    if (cw == 16)
+      return launch_sweep_cw<T, LOWER, NT, 16>(grid, shm, s, n, nrhs, LU, strideA, perm, B, strideB, X, strideX,
+                                               info, mask);
+  }
   if (cw == 8)
     return launch_sweep_cw<T, LOWER, NT, 8>(grid, shm, s, n, nrhs, LU, strideA, perm, B, strideB, X, strideX, info,
                                             mask);

@@ -78,7 +78,8 @@ __global__ __launch_bounds__(NTH) void leaf_kernel(LeafArgs<T> a) {
 
   // rows/columns left of a block: at most LEAF - IB (none when IB = LEAF)
   constexpr int LEFT = LEAF - IB > 0 ? LEAF - IB : 1;
-  __shared__ T L11[LEAF][LEAF + 1];
+  // L rows of the blocks before the last, for the later blocks' U solves
+  __shared__ T L11[LEFT][LEFT + 1];
   __shared__ T Ublk[LEFT][IB];
   __shared__ T red_v[2][NWL];  // candidate magnitudes (exact in T)
   __shared__ int red_p[2][NWL];
@@ -253,18 +254,22 @@ __global__ __launch_bounds__(NTH) void leaf_kernel(LeafArgs<T> a) {
     // ---- L11 rows of the block's new pivot rows: block columns from
     // registers, left columns (from their pre-block places) loaded by all
     // threads in parallel
+    // (only blocks before the last: rows and columns below LEFT)
+    const bool need_l11 = jb + ib < w;
     if (t == 0) mv_cnt = 0;
+    if (need_l11) {
 #pragma unroll
-    for (int s = 0; s < R; ++s) {
-      if (pos[s] >= jb && pos[s] < jb + ib) {
-        stage_pid[pos[s] - jb] = opos[s];
+      for (int s = 0; s < R; ++s) {
+        if (pos[s] >= jb && pos[s] < jb + ib) {
+          stage_pid[pos[s] - jb] = opos[s];
 #pragma unroll
-        for (int c = 0; c < IB; ++c)
-          if (c < ib) L11[pos[s]][jb + c] = v[s][c];
+          for (int c = 0; c < IB; ++c)
+            if (c < ib) L11[pos[s]][jb + c] = v[s][c];
+        }
       }
     }
     __syncthreads();
-    if (jb > 0) {
+    if (need_l11 && jb > 0) {
       constexpr int LIT = (IB * (LEAF - IB) + NTH - 1) / NTH > 0 ? (IB * (LEAF - IB) + NTH - 1) / NTH : 1;
       T l[LIT];
 #pragma unroll
@@ -563,7 +568,20 @@ struct Rec {
     // short panels: 256 threads (half of a 512-thread workgroup would hold
     // no rows, and the per-column barrier spans fewer waves); taller panels
     // measured faster with 512 threads and fewer rows per thread
-    if (m <= PT / 2) launch_leaf<T, 1, 64, PT / 2>(grid, shm, la);
+    // Many matrices (at least four per CU): one or two waves per matrix with
+    // two rows per lane (one up to 64 rows); the per-column pivot search,
+    // pivot-row exchange and barrier then cost one or two waves' instructions
+    // instead of four, and the other matrices' waves hide the column chain's
+    // latency. (FP64 above 128 rows measured faster with the 256-thread form.)
+    static const int thin_env = [] {
+      const char* e = std::getenv("NUMPP_HIP_LEAF_THIN");
+      return e ? std::atoi(e) : 1;
+    }();
+    const bool thin = thin_env != 0 && count >= 4 * (int64_t)std::max(cu_count(), 1);
+    if (thin && m <= 64) launch_leaf<T, 1, 64, 64>(grid, shm, la);
+    else if (thin && m <= 128) launch_leaf<T, 2, F ? 64 : 32, 64>(grid, shm, la);
+    else if (F && thin && m <= 256) launch_leaf<T, 2, 64, 128>(grid, shm, la);
+    else if (m <= PT / 2) launch_leaf<T, 1, 64, PT / 2>(grid, shm, la);
     else if (m <= PT) launch_leaf<T, 1, 64>(grid, shm, la);
     else if (m <= 2 * PT) launch_leaf<T, 2, F ? 64 : 16>(grid, shm, la);
     else if (m <= 4 * PT) launch_leaf<T, 4, F ? 32 : 8>(grid, shm, la);

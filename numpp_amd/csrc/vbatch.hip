@@ -256,7 +256,12 @@ int plan_build(Plan* p, int autotune, hipStream_t s) {
   // measured total time (sum over buckets of count x per-system time).
   std::vector<int> cands = {32, 64};
   int bestG = 32;
-  if (autotune) {
+  if (const char* e = std::getenv("NUMPP_HIP_VBATCH_G")) {
+    const int g = std::atoi(e);
+    if (g == 16 || g == 32 || g == 64) cands = {g};
+    bestG = cands[0];
+  }
+  if (autotune && cands.size() > 1) {
     double bestT = 1e300;
     for (int G : cands) {
       std::map<int, int64_t> hist;

@@ -55,6 +55,26 @@ def test_refined_converges_tall_panels(hip, precision):
     check_results(a, b, x, berr)
 
 
+@pytest.mark.parametrize("precision", ["fp32", "bf16"])
+def test_refined_many_systems(hip, precision):
+    # 1100 general systems of order 200 (at least four per CU: the panels run
+    # two waves per matrix); every backward error from the device report and
+    # recomputed here, a sample against the oracle
+    rng = np.random.default_rng(77)
+    count, n = 1100, 200
+    a = rng.uniform(-1, 1, (count, n, n))
+    b = rng.standard_normal((count, n, 2))
+    x, berr, iters, info = nla.solve_refined(a, b, precision=precision)
+    assert np.all(info == 0)
+    assert np.all(berr <= 1e-12), berr.max()
+    r = b - a @ x
+    eta = np.abs(r).max(axis=1) / (np.abs(a).sum(axis=2).max(axis=1)[:, None] * np.abs(x).max(axis=1)
+                                   + np.abs(b).max(axis=1))
+    assert eta.max() <= 1e-12, eta.max()
+    idx = np.arange(0, count, 157)
+    check_results(a[idx], b[idx], x[idx], berr[idx], cond=max(np.linalg.cond(a[i], 1) for i in idx))
+
+
 def test_fp64_precision_reports(hip):
     a, b = make_stack(7, 2, 128, 2, 1e4)
     x, berr, iters, info = nla.solve_refined(a, b, precision="fp64")

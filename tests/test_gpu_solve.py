@@ -129,6 +129,29 @@ def test_large_blocked_pivots_2100(hip):
     assert np.abs(x[0] - xo).max() <= 1e-13 * np.linalg.cond(a[0], 1) * np.abs(xo).max()
 
 
+@pytest.mark.parametrize("n", [100, 128, 200, 256])
+def test_many_systems_thin_leaves(hip, n):
+    # 1100 systems (at least four per CU): the panels run one or two waves per
+    # matrix. General matrices, so rows move at nearly every step; pivots of a
+    # sample against the oracle, every solution by its backward error.
+    rng = np.random.default_rng(40 + n)
+    count = 1100
+    a = rng.uniform(-1, 1, (count, n, n))
+    b = rng.standard_normal((count, n, 2))
+    lu, x, ipiv, info = gesv_host(a, b)
+    assert np.all(info == 0)
+    r = b - a @ x
+    eta = np.abs(r).max(axis=1) / (np.abs(a).sum(axis=2).max(axis=1)[:, None] * np.abs(x).max(axis=1)
+                                   + np.abs(b).max(axis=1))
+    assert eta.max() < 1e-14, eta.max()
+    for i in range(0, count, 137):
+        xo, luo, steps = oracle.solve(a[i], b[i], with_factors=True)
+        if pivot_gaps(a[i]) > 1e-6:
+            np.testing.assert_array_equal(ipiv[i], steps, err_msg=f"n={n} system {i}")
+            assert np.abs(lu[i] - luo).max() <= 1e-11 * np.abs(luo).max(), (n, i)
+        assert np.abs(x[i] - xo).max() <= 1e-13 * np.linalg.cond(a[i], 1) * np.abs(xo).max(), (n, i)
+
+
 def test_singular_exact_zero(hip):
     n = 70
     rng = np.random.default_rng(4)
