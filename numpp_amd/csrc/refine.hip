@@ -266,6 +266,113 @@ __global__ void residual_finish_kernel(int64_t count, int n, int nrhs, int nchun
   atomic_max_nonneg(rmax + mat * nrhs + (int)(e % nrhs), fabs(res));
 }
 
+// R = B - A X with the per-column maxima of |r| and |x|, for n <= 256: one
+// workgroup per (system, 16 right-hand-side columns). The X chunk is staged in
+// LDS (zero rows up to a multiple of 16, zero columns up to 16); each wave takes
+// 16-row tiles of A and accumulates them with FP64 MFMA 16x16x4, loading A
+// straight from global memory: a lane takes four consecutive entries of its row
+// per 16-column chunk (MFMA q of the chunk contracts over k = 4 lk + q, the
+// same k in the A and the X operand), four chunks in flight. A is read once;
+// R, the maxima and B's loads are fused into the same pass.
+constexpr int RSM_N = 256;
+template <bool VEC>
+__global__ __launch_bounds__(256) void residual_small_kernel(int n, int nrhs, const double* Ab, const double* Xb,
+                                                             const double* Bb, double* Rb, double* rmax,
+                                                             double* xmax, const int32_t* active) {
+  constexpr int KU = 4;  // 16-column chunks per step
+  const int64_t mat = blockIdx.x;
+  if (active && active[mat] != 1) return;
+  const int c0 = blockIdx.y * 16, nc = min(16, nrhs - c0);
+  const int64_t per = (int64_t)n * nrhs;
+  const double* __restrict__ A = Ab + mat * (int64_t)n * n;
+  const double* __restrict__ X = Xb + mat * per;
+  const double* __restrict__ B = Bb + mat * per;
+  double* __restrict__ R = Rb + mat * per;
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6, lr = lane & 15, lk = lane >> 4;
+  __shared__ double xs[RSM_N][16];
+  __shared__ double red_r[4][16], red_x[4][16];
+  const int np = (n + 15) & ~15;
+  double xm = 0.0;  // column t & 15
+  for (int e = t; e < np * 16; e += 256) {
+    const int k = e >> 4, j = e & 15;
+    const double v = (k < n && j < nc) ? X[(int64_t)k * nrhs + c0 + j] : 0.0;
+    xs[k][j] = v;
+    xm = fmax(xm, fabs(v));
+  }
+  xm = fmax(xm, __shfl_xor(xm, 16, 64));
+  xm = fmax(xm, __shfl_xor(xm, 32, 64));
+  if (lane < 16) red_x[wave][lane] = xm;
+  __syncthreads();
+  double rm = 0.0;  // column lr
+  for (int r0 = wave * 16; r0 < n; r0 += 64) {
+    const int row = min(r0 + lr, n - 1);
+    const double* __restrict__ ar = A + (int64_t)row * n;
+    // this lane's outputs: rows r0 + lk + 4 v, column lr
+    double bv[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int rr = min(r0 + lk + 4 * v, n - 1);
+      bv[v] = lr < nc ? B[(int64_t)rr * nrhs + c0 + lr] : 0.0;
+    }
+    v4d acc = {0.0, 0.0, 0.0, 0.0};
+    for (int k0 = 0; k0 < n; k0 += 16 * KU) {
+      double a[KU][4];
+#pragma unroll
+      for (int u = 0; u < KU; ++u) {
+        const int k = k0 + 16 * u + 4 * lk;
+        if constexpr (VEC) {
+          // n even: a pair starting at an even index <= n - 2 lies inside the
+          // row (and is 16-byte aligned); pairs past the row are clamped and
+          // their values zeroed
+          const double2 p0 = *reinterpret_cast<const double2*>(ar + min(k, n - 2));
+          const double2 p1 = *reinterpret_cast<const double2*>(ar + min(k + 2, n - 2));
+          a[u][0] = k < n ? p0.x : 0.0;
+          a[u][1] = k + 1 < n ? p0.y : 0.0;
+          a[u][2] = k + 2 < n ? p1.x : 0.0;
+          a[u][3] = k + 3 < n ? p1.y : 0.0;
+        } else {
+#pragma unroll
+          for (int q = 0; q < 4; ++q) a[u][q] = k + q < n ? ar[min(k + q, n - 1)] : 0.0;
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < KU; ++u) {
+        const int kb = k0 + 16 * u + 4 * lk;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          // chunks past np hold no staged rows: their A values are zero, and
+          // the X operand is read from row 0 instead (finite)
+          const int kx = kb + q < np ? kb + q : 0;
+          acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u][q], xs[kx][lr], acc, 0, 0, 0);
+        }
+      }
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int rr = r0 + lk + 4 * v;
+      if (rr < n && lr < nc) {
+        const double r = bv[v] - acc[v];
+        R[(int64_t)rr * nrhs + c0 + lr] = r;
+        rm = fmax(rm, fabs(r));
+      }
+    }
+  }
+  rm = fmax(rm, __shfl_xor(rm, 16, 64));
+  rm = fmax(rm, __shfl_xor(rm, 32, 64));
+  if (lane < 16) red_r[wave][lane] = rm;
+  __syncthreads();
+  if (t < nc) {
+    double r = red_r[0][t], x = red_x[0][t];
+#pragma unroll
+    for (int w = 1; w < 4; ++w) {
+      r = fmax(r, red_r[w][t]);
+      x = fmax(x, red_x[w][t]);
+    }
+    rmax[mat * nrhs + c0 + t] = r;
+    xmax[mat * nrhs + c0 + t] = x;
+  }
+}
+
 // Column maxima of R and X for small orders: one workgroup per system, 16
 // columns at a time, 16 row groups reduced through LDS. The residual itself
 // (R = B - A X) comes from the batched MFMA GEMM (alpha = -1, beta = 1 on a
@@ -364,6 +471,19 @@ inline dim3 g1(int64_t n, int b = 256) { return dim3((unsigned)ceil_div(n, b)); 
 // Residual + per-(system, column) maxima of |r| and |x| for the current X.
 int residual_launch(int64_t count, int n, int nrhs, const double* A, const double* X, const double* B, double* R,
                     double* rmax, double* xmax, const int32_t* active, hipStream_t s) {
+  if (n <= RSM_N && count <= 2147483647LL && ceil_div(nrhs, 16) <= 65535) {
+    ProfScope ps(kProfResidual, 2.0 * count * n * n * nrhs, 8.0 * count * ((double)n * n + 3.0 * n * nrhs), s);
+    dim3 grid((unsigned)count, (unsigned)ceil_div(nrhs, 16));
+    const bool vec = (n % 2 == 0) && n >= 4 && (reinterpret_cast<uintptr_t>(A) % 16 == 0);
+    if (vec)
+      hipLaunchKernelGGL(residual_small_kernel<true>, grid, dim3(256), 0, s, n, nrhs, A, X, B, R, rmax, xmax,
+                         active);
+    else
+      hipLaunchKernelGGL(residual_small_kernel<false>, grid, dim3(256), 0, s, n, nrhs, A, X, B, R, rmax, xmax,
+                         active);
+    NPPH_HIP(hipGetLastError());
+    return NPPH_OK;
+  }
   if (n <= RS_N && count <= 65535) {
     ProfScope ps(kProfResidual, 2.0 * count * n * n * nrhs, 8.0 * count * ((double)n * n + 3.0 * n * nrhs), s);
     NPPH_HIP(hipMemcpyAsync(R, B, sizeof(double) * (size_t)count * n * nrhs, hipMemcpyDeviceToDevice, s));

@@ -55,6 +55,21 @@ def test_refined_converges_tall_panels(hip, precision):
     check_results(a, b, x, berr)
 
 
+@pytest.mark.parametrize("n,nrhs", [(5, 17), (33, 1), (255, 40), (256, 16)])
+def test_refined_small_residual_shapes(hip, n, nrhs):
+    # the fused small-order residual: odd orders (scalar A loads), orders not a
+    # multiple of 16, and more than 16 right-hand sides (several column chunks)
+    a, b = make_stack(500 + n + nrhs, 3, n, nrhs, 1e4)
+    x, berr, iters, info = nla.solve_refined(a, b, precision="fp32")
+    assert np.all(info == 0)
+    check_results(a, b, x, berr)
+    for i in range(a.shape[0]):
+        eta = oracle.backward_error(a[i], b[i], x[i])
+        # the device's eta (FP64 MFMA residual, rounding error up to about
+        # n u |A||x|) matches the oracle's (Dot2 residual)
+        assert np.all(np.abs(berr[i] - eta) <= 0.5 * eta + 2e-14), (berr[i], eta)
+
+
 @pytest.mark.parametrize("precision", ["fp32", "bf16"])
 def test_refined_many_systems(hip, precision):
     # 1100 general systems of order 200 (at least four per CU: the panels run
