@@ -16,6 +16,7 @@
 // device would need L2 write-back and invalidation at device scope).
 #include <algorithm>
 #include <cstdlib>
+#include <type_traits>
 
 #include "common.hpp"
 #include "lu.hpp"
@@ -26,6 +27,16 @@ namespace npph {
 namespace {
 
 constexpr int GB = 64;  // rows per block
+
+__device__ __forceinline__ v4f mfma16(float a, float b, v4f c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ v4d mfma16(double a, double b, v4d c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+// output row of accumulator element v for lane group lk (16x16x4 layouts)
+template <class T>
+__device__ __forceinline__ int mfma_row(int lk, int v) { return sizeof(T) == 4 ? 4 * lk + v : lk + 4 * v; }
 
 template <class T, bool LOWER, int NT, int CW>
 __global__ __launch_bounds__(NT) void sweep_kernel(int n, int nrhs, const T* LUb, int64_t strideA,
@@ -96,6 +107,60 @@ __global__ __launch_bounds__(NT) void sweep_kernel(int n, int nrhs, const T* LUb
       const int e = t + u * NT, rr = e / GB, cc = e - rr * GB;
       dpre[u] = LU[(int64_t)(r0 + min(rr, bs - 1)) * n + r0 + min(cc, bs - 1)];
     }
+    if constexpr (NT == 256) {
+      // ---- off-diagonal contributions as a (64 x K) x (K x 16) product on
+      // MFMA 16x16x4: wave w takes block rows 16 w .. 16 w + 15; a lane loads
+      // four consecutive factor entries of its row per 16-column chunk (MFMA q
+      // of the chunk contracts over k = 4 lk + q in both operands), four
+      // chunks in flight; the X operand comes from the LDS slice (columns
+      // past CW are zero)
+      typedef std::conditional_t<sizeof(T) == 4, v4f, v4d> acc_t;
+      constexpr int KU = 4;
+      const int lr = lane & 15, lk = lane >> 4, rw = 16 * wave;
+      acc_t acc = {};
+      if (rw < bs && klo < khi) {
+        const T* __restrict__ Lr = LU + (int64_t)(r0 + min(rw + lr, bs - 1)) * n;
+        // aligned vector loads when every row starts on 16 bytes
+        const bool vec = n % (16 / (int)sizeof(T)) == 0 && n >= 4 && reinterpret_cast<uintptr_t>(LUb) % 16 == 0 &&
+                         strideA % (16 / (int)sizeof(T)) == 0;
+        for (int kb = klo; kb < khi; kb += 16 * KU) {
+          T a[KU][4];
+#pragma unroll
+          for (int u = 0; u < KU; ++u) {
+            const int k = kb + 16 * u + 4 * lk;
+            if (vec) {
+              // k and n are multiples of 4 (FP32) or 2 (FP64): the clamped
+              // vectors lie inside the row
+              if constexpr (sizeof(T) == 4) {
+                const float4 p = *reinterpret_cast<const float4*>(Lr + min(k, n - 4));
+                a[u][0] = p.x; a[u][1] = p.y; a[u][2] = p.z; a[u][3] = p.w;
+              } else {
+                const double2 p0 = *reinterpret_cast<const double2*>(Lr + min(k, n - 2));
+                const double2 p1 = *reinterpret_cast<const double2*>(Lr + min(k + 2, n - 2));
+                a[u][0] = p0.x; a[u][1] = p0.y; a[u][2] = p1.x; a[u][3] = p1.y;
+              }
+            } else {
+#pragma unroll
+              for (int q = 0; q < 4; ++q) a[u][q] = Lr[min(k + q, khi - 1)];
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) a[u][q] = k + q < khi ? a[u][q] : T(0);
+          }
+#pragma unroll
+          for (int u = 0; u < KU; ++u)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+              const int kx = min(kb + 16 * u + 4 * lk + q, khi - 1);
+              const T b = lr < CW ? xs[(int64_t)kx * CW + (lr < CW ? lr : 0)] : T(0);
+              acc = mfma16(a[u][q], b, acc);
+            }
+        }
+      }
+      if (lr < CW) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) part[rw + mfma_row<T>(lk, v)][lr] = acc[v];
+      }
+    } else {
     // ---- off-diagonal contributions: row r of the block; its S threads take
     // interleaved columns (klo + sg + S m), so a row's threads read
     // consecutive factor entries and consecutive rows of the LDS slice
@@ -129,6 +194,7 @@ __global__ __launch_bounds__(NT) void sweep_kernel(int n, int nrhs, const T* LUb
     if (sg == 0) {
 #pragma unroll
       for (int c = 0; c < CW; ++c) part[r][c] = acc[c];
+    }
     }
 #pragma unroll
     for (int u = 0; u < DPT; ++u) {
