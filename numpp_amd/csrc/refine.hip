@@ -99,10 +99,13 @@ __global__ void fill_kernel(double* p, int64_t n, double v) {
   if (i < n) p[i] = v;
 }
 
-__global__ void init_state_kernel(int64_t count, const int32_t* info, int32_t* active, int32_t* iters) {
+// n_stop counts the systems set to 2 (stopped without convergence).
+__global__ void init_state_kernel(int64_t count, const int32_t* info, int32_t* active, int32_t* iters,
+                                  int32_t* n_stop) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= count) return;
   active[i] = info[i] == 0 ? 1 : 2;  // a low-precision zero pivot goes to the FP64 fallback
+  if (info[i] != 0 && n_stop) atomicAdd(n_stop, 1);
   iters[i] = 0;
 }
 
@@ -415,10 +418,11 @@ __global__ __launch_bounds__(256) void colmax_kernel(int n, int nrhs, const doub
 // Convergence test per system (one thread per system). Records eta per column,
 // decides active (1 = continue, 0 = converged, 2 = stopped without
 // convergence), computes the next sweep's scales, clears the max buffers and
-// counts systems still active.
+// counts systems still active (n_active) and newly stopped (n_stop).
 __global__ void eta_kernel(int64_t count, int nrhs, double tol, int max_iter, const double* anorm,
                            const double* bmax, double* rmax, double* xmax, double* scale, double* prev,
-                           double* berr, int32_t* active, const int32_t* iters, int32_t* n_active) {
+                           double* berr, int32_t* active, const int32_t* iters, int32_t* n_active,
+                           int32_t* n_stop) {
   const int64_t mat = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (mat >= count || active[mat] != 1) return;
   double worst = 0.0;
@@ -437,6 +441,7 @@ __global__ void eta_kernel(int64_t count, int nrhs, double tol, int max_iter, co
     active[mat] = 0;
   } else if (iters[mat] >= max_iter || !(worst < 0.5 * prev[mat])) {
     active[mat] = 2;
+    if (n_stop) atomicAdd(n_stop, 1);
   } else {
     atomicAdd(n_active, 1);
   }
@@ -449,13 +454,36 @@ __global__ void scale_from_bmax_kernel(int64_t n, const double* bmax, double* sc
 }
 
 // Systems that did not converge (active == 2): mask for the FP64 fallback.
-__global__ void fallback_mask_kernel(int64_t count, const int32_t* active, int32_t* mask, int32_t* iters,
-                                     int32_t* n_fb) {
+__global__ void fallback_mask_kernel(int64_t count, const int32_t* active, int32_t* mask, int32_t* iters) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= count) return;
   const int fb = active[i] == 2 ? 1 : 0;
   mask[i] = fb;
-  if (fb) { iters[i] = -1; atomicAdd(n_fb, 1); }
+  if (fb) iters[i] = -1;
+}
+
+// Per host thread: pinned flags for the convergence read-back and the event
+// that marks their arrival (kept until the process ends).
+struct FlagSlot {
+  int32_t* host = nullptr;
+  hipEvent_t ev = nullptr;
+};
+int flag_slot(FlagSlot** out) {
+  thread_local FlagSlot slot;
+  if (!slot.host) {
+    void* p = nullptr;
+    NPPH_HIP(hipHostMalloc(&p, 64, hipHostMallocDefault));
+    hipEvent_t e = nullptr;
+    const hipError_t err = hipEventCreateWithFlags(&e, hipEventDisableTiming);
+    if (err != hipSuccess) {
+      (void)hipHostFree(p);
+      NPPH_HIP(err);
+    }
+    slot.host = static_cast<int32_t*>(p);
+    slot.ev = e;
+  }
+  *out = &slot;
+  return NPPH_OK;
 }
 
 __global__ void copy_masked_kernel(int64_t count, int64_t per, const double* src, double* dst, const int32_t* mask) {
@@ -629,20 +657,22 @@ int refine_solve(const npph_refine_opts& o, const double* A, const RefineFactors
   auto report = [&]() -> int {
     NPPH_TRY(residual_launch(count, n, nrhs, A, X, B, R, rmax, xmax, active, s));
     hipLaunchKernelGGL(eta_kernel, g1(count), dim3(256), 0, s, count, nrhs, INFINITY, 0, anorm, bmax, rmax, xmax,
-                       scale, prev, berr_p, active, iters, ctr);
+                       scale, prev, berr_p, active, iters, ctr, nullptr);
     NPPH_HIP(hipGetLastError());
     return NPPH_OK;
   };
 
   if (o.precision == NPPH_PREC_FP64) {
     NPPH_TRY(blocked_getrs_perm<double>(count, n, nrhs, f.lu64, per_a, f.perm, B, per_b, X, per_b, f.info, nullptr, s));
-    hipLaunchKernelGGL(init_state_kernel, g1(count), dim3(256), 0, s, count, info, active, iters);
+    hipLaunchKernelGGL(init_state_kernel, g1(count), dim3(256), 0, s, count, info, active, iters, nullptr);
     NPPH_TRY(report());
   } else {
     float *Rlo, *Dlo;
     NPPH_TRY(workspace(kWsRlo, sizeof(float) * (size_t)nb_all, (void**)&Rlo));
     NPPH_TRY(workspace(kWsD, sizeof(float) * (size_t)nb_all, (void**)&Dlo));
-    hipLaunchKernelGGL(init_state_kernel, g1(count), dim3(256), 0, s, count, info, active, iters);
+    FlagSlot* fs;
+    NPPH_TRY(flag_slot(&fs));
+    hipLaunchKernelGGL(init_state_kernel, g1(count), dim3(256), 0, s, count, info, active, iters, ctr + 1);
     hipLaunchKernelGGL(scale_from_bmax_kernel, g1(nk), dim3(256), 0, s, nk, bmax, scale);
     // Sweep 0 (x = 0, r = b).
     hipLaunchKernelGGL(scale_demote_kernel, g1(nb_all), dim3(256), 0, s, count, n, nrhs, B, scale, Rlo, active);
@@ -652,29 +682,36 @@ int refine_solve(const npph_refine_opts& o, const double* A, const RefineFactors
     hipLaunchKernelGGL(count_sweep_kernel, g1(count), dim3(256), 0, s, count, active, iters);
     NPPH_HIP(hipGetLastError());
     const int max_iter = o.max_iter > 0 ? o.max_iter : 10;
-    for (;;) {
+    // Each test's counts (systems still active, systems stopped so far) come
+    // back through pinned memory while the next sweep is already queued: that
+    // sweep only touches systems still active, so when none is left it changes
+    // nothing, and the device never idles for the host's round trip.
+    for (int it = 0;; ++it) {
       NPPH_TRY(residual_launch(count, n, nrhs, A, X, B, R, rmax, xmax, active, s));
       hipLaunchKernelGGL(zero_int_kernel, dim3(1), dim3(64), 0, s, ctr, 1);
       hipLaunchKernelGGL(eta_kernel, g1(count), dim3(256), 0, s, count, nrhs, o.tol, max_iter, anorm, bmax, rmax,
-                         xmax, scale, prev, berr_p, active, iters, ctr);
+                         xmax, scale, prev, berr_p, active, iters, ctr, ctr + 1);
       NPPH_HIP(hipGetLastError());
-      int32_t n_act = 0;
-      NPPH_HIP(hipMemcpyAsync(&n_act, ctr, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-      NPPH_HIP(hipStreamSynchronize(s));
-      if (n_act == 0) break;
-      hipLaunchKernelGGL(scale_demote_kernel, g1(nb_all), dim3(256), 0, s, count, n, nrhs, R, scale, Rlo, active);
-      NPPH_TRY(blocked_getrs_perm<float>(count, n, nrhs, f.lu32, per_a, f.perm, Rlo, per_b, Dlo, per_b, nullptr,
-                                         active, s));
-      hipLaunchKernelGGL(update_kernel, g1(nb_all), dim3(256), 0, s, count, n, nrhs, X, Dlo, scale, active, 0);
-      hipLaunchKernelGGL(count_sweep_kernel, g1(count), dim3(256), 0, s, count, active, iters);
-      NPPH_HIP(hipGetLastError());
+      NPPH_HIP(hipMemcpyAsync(fs->host, ctr, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+      NPPH_HIP(hipEventRecord(fs->ev, s));
+      // eta stops every system after max_iter sweeps, so the counts reach
+      // zero by then; the bound only guards the loop
+      const bool more = it <= max_iter;
+      if (more) {
+        hipLaunchKernelGGL(scale_demote_kernel, g1(nb_all), dim3(256), 0, s, count, n, nrhs, R, scale, Rlo, active);
+        NPPH_TRY(blocked_getrs_perm<float>(count, n, nrhs, f.lu32, per_a, f.perm, Rlo, per_b, Dlo, per_b, nullptr,
+                                           active, s));
+        hipLaunchKernelGGL(update_kernel, g1(nb_all), dim3(256), 0, s, count, n, nrhs, X, Dlo, scale, active, 0);
+        hipLaunchKernelGGL(count_sweep_kernel, g1(count), dim3(256), 0, s, count, active, iters);
+        NPPH_HIP(hipGetLastError());
+      }
+      NPPH_HIP(hipEventSynchronize(fs->ev));
+      if (fs->host[0] == 0 || !more) break;
     }
+    const int32_t n_fb = fs->host[1];
     if (o.fallback_fp64) {
-      hipLaunchKernelGGL(fallback_mask_kernel, g1(count), dim3(256), 0, s, count, active, mask, iters, ctr + 1);
-      int32_t n_fb = 0;
-      NPPH_HIP(hipMemcpyAsync(&n_fb, ctr + 1, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-      NPPH_HIP(hipStreamSynchronize(s));
       if (n_fb > 0) {
+        hipLaunchKernelGGL(fallback_mask_kernel, g1(count), dim3(256), 0, s, count, active, mask, iters);
         NPPH_TRY(fp64_solve(true));
         NPPH_HIP(hipMemcpyAsync(active, mask, sizeof(int32_t) * count, hipMemcpyDeviceToDevice, s));
         NPPH_TRY(report());
