@@ -80,7 +80,10 @@ struct RefineFactors {
 size_t refine_factor_bytes(int precision, int64_t count, int n);
 void refine_factor_carve(int precision, int64_t count, int n, void* buf, RefineFactors& f);
 // Factorises A (count x n x n FP64, unchanged) into f in f.precision.
-int refine_factor(const npph_refine_opts& o, const double* A, RefineFactors& f, hipStream_t s);
+// prepped: the caller has already written A's working copy (f.lu64 or the
+// FP32 rounding in f.lu32) and ||A||_inf per system into f.anorm.
+int refine_factor(const npph_refine_opts& o, const double* A, RefineFactors& f, hipStream_t s,
+                  bool prepped = false);
 // Solves with resident factors, refining in FP64 against A (needed for the
 // residual) until every column meets o.tol; B and X must not alias.
 int refine_solve(const npph_refine_opts& o, const double* A, const RefineFactors& f, int nrhs, const double* B,

@@ -564,16 +564,18 @@ void refine_factor_carve(int precision, int64_t count, int n, void* buf, RefineF
   f.info = (int32_t*)take(sizeof(int32_t) * (size_t)count);
 }
 
-int refine_factor(const npph_refine_opts& o, const double* A, RefineFactors& f, hipStream_t s) {
+int refine_factor(const npph_refine_opts& o, const double* A, RefineFactors& f, hipStream_t s, bool prepped) {
   const int64_t count = f.count;
   const int n = f.n;
   if (count <= 0 || n <= 0) return NPPH_OK;
   if (count > 65535) { set_error("refined solve: count > 65535"); return NPPH_ERR_INVALID; }
-  hipLaunchKernelGGL(fill_kernel, g1(count), dim3(256), 0, s, f.anorm, count, 0.0);
+  if (!prepped) hipLaunchKernelGGL(fill_kernel, g1(count), dim3(256), 0, s, f.anorm, count, 0.0);
   if (o.precision == NPPH_PREC_FP64) {
     const int64_t per_a = (int64_t)n * n;
-    hipLaunchKernelGGL(rowabs_kernel, g1(count * n * 64), dim3(256), 0, s, count, n, A, f.anorm);
-    NPPH_HIP(hipMemcpyAsync(f.lu64, A, sizeof(double) * count * per_a, hipMemcpyDeviceToDevice, s));
+    if (!prepped) {
+      hipLaunchKernelGGL(rowabs_kernel, g1(count * n * 64), dim3(256), 0, s, count, n, A, f.anorm);
+      NPPH_HIP(hipMemcpyAsync(f.lu64, A, sizeof(double) * count * per_a, hipMemcpyDeviceToDevice, s));
+    }
     if (n <= 64) {
       UniformBatch u{count, n, f.lu64, per_a, f.lu64, f.ipiv, f.info, 0, nullptr, 0, nullptr};
       NPPH_TRY(small_lu_launch(u, s));
@@ -589,7 +591,7 @@ int refine_factor(const npph_refine_opts& o, const double* A, RefineFactors& f, 
     return NPPH_ERR_INVALID;
   }
   const int terms = o.precision == NPPH_PREC_BF16 ? (o.bf16_terms > 0 ? o.bf16_terms : 6) : 0;
-  hipLaunchKernelGGL(demote_kernel, g1(count * n * 64), dim3(256), 0, s, count, n, A, f.lu32, f.anorm);
+  if (!prepped) hipLaunchKernelGGL(demote_kernel, g1(count * n * 64), dim3(256), 0, s, count, n, A, f.lu32, f.anorm);
   NPPH_HIP(hipGetLastError());
   return blocked_getrf_perm<float>(count, n, f.lu32, (int64_t)n * n, f.ipiv, f.perm, f.info, terms, nullptr, s);
 }

@@ -77,24 +77,50 @@ struct Plan {
 };
 
 // ---- pack / unpack ----
-__global__ __launch_bounds__(256) void pack_kernel(int N, int R, const int32_t* ids, const int32_t* nn,
-                                                   const int32_t* nr, const int64_t* aoff, const int64_t* boff,
-                                                   const double* A, const double* B, double* Ap, double* Bp) {
+
+// Packs one bucket and prepares its factorisation in the same pass: the padded
+// FP64 copy (kept for the residual), the working copy the LU overwrites (FP64,
+// or rounded to FP32 for the mixed-precision factorisation) and ||A||_inf of
+// the system itself (its n rows). One workgroup per system, one wave per row
+// (lanes over columns); then B's padded copy (zero rows and columns).
+template <class LO>
+__global__ __launch_bounds__(256) void pack_prep_kernel(int N, int R, const int32_t* ids, const int32_t* nn,
+                                                        const int32_t* nr, const int64_t* aoff, const int64_t* boff,
+                                                        const double* A, const double* B, double* Ap, double* Bp,
+                                                        LO* Lo, double* anorm) {
   const int64_t k = blockIdx.x;
   const int id = ids[k];
   const int n = nn[id], m = nr[id];
-  const double* a = A + aoff[id];
-  const double* b = B + boff[id];
-  double* ap = Ap + k * (int64_t)N * N;
-  double* bp = Bp + k * (int64_t)N * R;
-  for (int e = threadIdx.x; e < N * N; e += 256) {
-    const int i = e / N, j = e - i * N;
-    ap[e] = (i < n && j < n) ? a[(int64_t)i * n + j] : (i == j ? 1.0 : 0.0);
+  const double* __restrict__ a = A + aoff[id];
+  const double* __restrict__ b = B + boff[id];
+  double* __restrict__ ap = Ap + k * (int64_t)N * N;
+  LO* __restrict__ lo = Lo + k * (int64_t)N * N;
+  double* __restrict__ bp = Bp + k * (int64_t)N * R;
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  __shared__ double red[4];
+  double amax = 0.0;
+  for (int i = wave; i < N; i += 4) {
+    const double* __restrict__ ar = a + (int64_t)min(i, n - 1) * n;
+    double s = 0.0;
+#pragma unroll 4
+    for (int j = lane; j < N; j += 64) {
+      const bool in = i < n && j < n;
+      const double v = in ? ar[min(j, n - 1)] : (i == j ? 1.0 : 0.0);
+      s += in ? fabs(v) : 0.0;
+      ap[(int64_t)i * N + j] = v;
+      lo[(int64_t)i * N + j] = (LO)v;
+    }
+#pragma unroll
+    for (int q = 32; q >= 1; q >>= 1) s += __shfl_xor(s, q, 64);
+    amax = fmax(amax, s);
   }
-  for (int e = threadIdx.x; e < N * R; e += 256) {
+  if (lane == 0) red[wave] = amax;
+  for (int e = t; e < N * R; e += 256) {
     const int i = e / R, c = e - i * R;
     bp[e] = (i < n && c < m) ? b[(int64_t)i * m + c] : 0.0;
   }
+  __syncthreads();
+  if (t == 0) anorm[k] = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
 }
 
 __global__ __launch_bounds__(256) void unpack_kernel(int N, int R, const int32_t* ids, const int32_t* nn,
@@ -302,12 +328,21 @@ int run_plan(Plan* p, const double* A, const double* B, double* X, double* berr,
              hipStream_t s) {
   for (auto& b : p->buckets) {
     const int64_t c = (int64_t)b.ids.size();
-    hipLaunchKernelGGL(pack_kernel, dim3((unsigned)c), dim3(256), 0, s, b.N, b.R, b.d_ids, p->d_n, p->d_nrhs,
-                       p->d_aoff, p->d_boff, A, B, b.A, b.B);
+    void* buf;
+    NPPH_TRY(workspace(kWsLo, refine_factor_bytes(p->opts.precision, c, b.N), &buf));
+    RefineFactors f;
+    refine_factor_carve(p->opts.precision, c, b.N, buf, f);
+    if (p->opts.precision == NPPH_PREC_FP64)
+      hipLaunchKernelGGL(pack_prep_kernel<double>, dim3((unsigned)c), dim3(256), 0, s, b.N, b.R, b.d_ids, p->d_n,
+                         p->d_nrhs, p->d_aoff, p->d_boff, A, B, b.A, b.B, f.lu64, f.anorm);
+    else
+      hipLaunchKernelGGL(pack_prep_kernel<float>, dim3((unsigned)c), dim3(256), 0, s, b.N, b.R, b.d_ids, p->d_n,
+                         p->d_nrhs, p->d_aoff, p->d_boff, A, B, b.A, b.B, f.lu32, f.anorm);
     NPPH_HIP(hipGetLastError());
     // FP64 runs LU + getrs + the backward-error report (iters = 0); the other
     // precisions refine.
-    NPPH_TRY(refined_impl(p->opts, c, b.N, b.R, b.A, b.B, b.X, b.berr, b.iters, b.info, s));
+    NPPH_TRY(refine_factor(p->opts, b.A, f, s, true));
+    NPPH_TRY(refine_solve(p->opts, b.A, f, b.R, b.B, b.X, b.berr, b.iters, b.info, s));
     hipLaunchKernelGGL(unpack_kernel, dim3((unsigned)c), dim3(256), 0, s, b.N, b.R, b.d_ids, p->d_n, p->d_nrhs,
                        p->d_boff, p->d_coff, b.X, b.info, b.berr, b.iters, X, info,
                        berr, iters);
