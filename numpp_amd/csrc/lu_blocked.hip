@@ -470,9 +470,9 @@ __global__ __launch_bounds__(256) void move_rows_fused_kernel(T* Abase, int64_t 
 // columns [c0, c0+nc). One thread per column (TT threads share one copy of L,
 // stored transposed in LDS), column-oriented substitution: step p subtracts
 // L[q][p] x[p] from every later row q, so the dependent chain is one FMA per
-// step rather than a q-term dot product per row.
-constexpr int TT = 256;
-template <class T>
+// step rather than a q-term dot product per row. TT = 64, 128 or 256 threads,
+// the narrowest that covers nc (idle threads would hold registers and LDS).
+template <class T, int TT>
 __global__ __launch_bounds__(TT) void trsm_leaf_kernel(T* Abase, int64_t strideA, int n, int r0, int s, int c0,
                                                        int nc, const int32_t* mask) {
   const int64_t mat = blockIdx.y;
@@ -569,19 +569,24 @@ struct Rec {
     // no rows, and the per-column barrier spans fewer waves); taller panels
     // measured faster with 512 threads and fewer rows per thread
     // Many matrices (at least four per CU): one or two waves per matrix with
-    // two rows per lane (one up to 64 rows); the per-column pivot search,
-    // pivot-row exchange and barrier then cost one or two waves' instructions
-    // instead of four, and the other matrices' waves hide the column chain's
-    // latency. (FP64 above 128 rows measured faster with the 256-thread form.)
+    // up to two rows per lane; the per-column pivot search, pivot-row exchange
+    // and barrier then cost one or two waves' instructions instead of four,
+    // and the other matrices' waves hide the column chain's latency.
     static const int thin_env = [] {
       const char* e = std::getenv("NUMPP_HIP_LEAF_THIN");
       return e ? std::atoi(e) : 1;
     }();
     const bool thin = thin_env != 0 && count >= 4 * (int64_t)std::max(cu_count(), 1);
+    // FP64 rows take twice the registers: above 64 rows it keeps one row per
+    // thread up to 128 rows and 16-column inner blocks up to 256 (measured).
     if (thin && m <= 64) launch_leaf<T, 1, 64, 64>(grid, shm, la);
-    else if (thin && m <= 128) launch_leaf<T, 2, F ? 64 : 32, 64>(grid, shm, la);
-    else if (F && thin && m <= 256) launch_leaf<T, 2, 64, 128>(grid, shm, la);
-    else if (m <= PT / 2) launch_leaf<T, 1, 64, PT / 2>(grid, shm, la);
+    else if (thin && m <= 128) {
+      if constexpr (F) launch_leaf<T, 2, 64, 64>(grid, shm, la);
+      else launch_leaf<T, 1, 64, 128>(grid, shm, la);
+    } else if (thin && m <= 256) {
+      if constexpr (F) launch_leaf<T, 2, 64, 128>(grid, shm, la);
+      else launch_leaf<T, 2, 16, 128>(grid, shm, la);
+    } else if (m <= PT / 2) launch_leaf<T, 1, 64, PT / 2>(grid, shm, la);
     else if (m <= PT) launch_leaf<T, 1, 64>(grid, shm, la);
     else if (m <= 2 * PT) launch_leaf<T, 2, F ? 64 : 16>(grid, shm, la);
     else if (m <= 4 * PT) launch_leaf<T, 4, F ? 32 : 8>(grid, shm, la);
@@ -631,9 +636,16 @@ struct Rec {
   int trsm(int r0, int sz, int c0, int nc) {
     if (nc <= 0) return NPPH_OK;
     if (sz <= LEAF) {
-      dim3 grid((unsigned)ceil_div(nc, TT), (unsigned)count);
       ProfScope ps(kProfMoveTrsm, (double)count * sz * sz * nc, 2.0 * sizeof(T) * count * (double)sz * nc, s);
-      hipLaunchKernelGGL(trsm_leaf_kernel<T>, grid, dim3(TT), 0, s, A, strideA, n, r0, sz, c0, nc, mask);
+      if (nc <= 64)
+        hipLaunchKernelGGL((trsm_leaf_kernel<T, 64>), dim3(1u, (unsigned)count), dim3(64), 0, s, A, strideA, n, r0,
+                           sz, c0, nc, mask);
+      else if (nc <= 128)
+        hipLaunchKernelGGL((trsm_leaf_kernel<T, 128>), dim3(1u, (unsigned)count), dim3(128), 0, s, A, strideA, n,
+                           r0, sz, c0, nc, mask);
+      else
+        hipLaunchKernelGGL((trsm_leaf_kernel<T, 256>), dim3((unsigned)ceil_div(nc, 256), (unsigned)count),
+                           dim3(256), 0, s, A, strideA, n, r0, sz, c0, nc, mask);
       NPPH_HIP(hipGetLastError());
       return NPPH_OK;
     }

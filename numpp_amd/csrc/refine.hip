@@ -56,13 +56,30 @@ __global__ __launch_bounds__(256) void demote_kernel(int64_t count, int n, const
   if (lane == 0) atomic_max_nonneg(anorm + mat, s);
 }
 
-// ||b_j||_inf per (matrix, column) and the power-of-two scale for sweep 0.
-__global__ void bnorm_kernel(int64_t count, int n, int nrhs, const double* B, double* bmax) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= count * n * nrhs) return;
-  const int64_t mat = i / ((int64_t)n * nrhs);
-  const int c = (int)(i % nrhs);
-  atomic_max_nonneg(bmax + mat * nrhs + c, fabs(B[i]));
+// ||b_j||_inf per (matrix, column): one workgroup per matrix, CPW columns
+// (nrhs rounded up to a power of two, at most 64) by 256 / CPW rows per pass,
+// reduced across a wave's row groups with xor shuffles and across the four
+// waves in LDS; every maximum is written once (no atomics).
+__global__ __launch_bounds__(256) void bnorm_kernel(int n, int nrhs, const double* Bb, double* bmax) {
+  const int64_t mat = blockIdx.x;
+  const double* __restrict__ B = Bb + mat * (int64_t)n * nrhs;
+  __shared__ double red[4][64];
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  int cpw = 1;
+  while (cpw < nrhs && cpw < 64) cpw <<= 1;
+  const int rp = 256 / cpw, r = t / cpw, cl = t % cpw;
+  for (int c0 = 0; c0 < nrhs; c0 += cpw) {
+    const int c = c0 + cl;
+    double m = 0.0;
+    if (c < nrhs)
+      for (int i = r; i < n; i += rp) m = fmax(m, fabs(B[(int64_t)i * nrhs + c]));
+    for (int o = cpw; o < 64; o <<= 1) m = fmax(m, __shfl_xor(m, o, 64));
+    if (lane < cpw) red[wave][lane] = m;
+    __syncthreads();
+    if (t < cpw && c0 + t < nrhs)
+      bmax[mat * nrhs + c0 + t] = fmax(fmax(red[0][t], red[1][t]), fmax(red[2][t], red[3][t]));
+    __syncthreads();
+  }
 }
 
 // ||A||_inf per matrix only (plain FP64 path, for the reported eta).
@@ -629,7 +646,7 @@ int refine_solve(const npph_refine_opts& o, const double* A, const RefineFactors
   hipLaunchKernelGGL(fill_kernel, g1(count), dim3(256), 0, s, prev, count, (double)INFINITY);
   hipLaunchKernelGGL(fill_kernel, g1(nk), dim3(256), 0, s, berr_p, nk, (double)NAN);
   hipLaunchKernelGGL(zero_int_kernel, dim3(1), dim3(64), 0, s, ctr, 2);
-  hipLaunchKernelGGL(bnorm_kernel, g1(nb_all), dim3(256), 0, s, count, n, nrhs, B, bmax);
+  hipLaunchKernelGGL(bnorm_kernel, dim3((unsigned)count), dim3(256), 0, s, n, nrhs, B, bmax);
   NPPH_HIP(hipMemcpyAsync(info, f.info, sizeof(int32_t) * count, hipMemcpyDeviceToDevice, s));
   NPPH_HIP(hipGetLastError());
 
