@@ -53,7 +53,7 @@ int workspace(int slot, size_t bytes, void** out);
 uint64_t workspace_generation();
 // True while `s` is being captured into a hipGraph (no allocation, no sync).
 bool stream_capturing(hipStream_t s);
-enum WsSlot { kWsA = 0, kWsB, kWsPiv, kWsInfo, kWsLo, kWsR, kWsX, kWsNorm, kWsMisc, kWsMove, kWsPerm, kWsD, kWsRlo, kWsNet, kWsFlags, kWsPart, kWsCount };
+enum WsSlot { kWsA = 0, kWsB, kWsPiv, kWsInfo, kWsLo, kWsR, kWsX, kWsNorm, kWsMisc, kWsMove, kWsPerm, kWsD, kWsNet, kWsFlags, kWsPart, kWsCount };
 
 // A private set of workspace slots. A thread that binds one (ws_bind) gets its
 // own buffers from workspace(), so calls running concurrently on different

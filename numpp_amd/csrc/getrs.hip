@@ -28,6 +28,14 @@ namespace {
 
 constexpr int GB = 64;  // rows per block
 
+// FP64 side of a refinement correction (unused by plain solves)
+struct RefIO {
+  const double* R = nullptr;      // residual (the forward sweep's right-hand side)
+  const double* scale = nullptr;  // per (system, column) power-of-two scale
+  double* X = nullptr;            // iterate: X (+)= correction / scale
+  int first = 0;                  // 1: X = correction / scale
+};
+
 __device__ __forceinline__ v4f mfma16(float a, float b, v4f c) {
   return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
@@ -38,10 +46,14 @@ __device__ __forceinline__ v4d mfma16(double a, double b, v4d c) {
 template <class T>
 __device__ __forceinline__ int mfma_row(int lk, int v) { return sizeof(T) == 4 ? 4 * lk + v : lk + 4 * v; }
 
-template <class T, bool LOWER, int NT, int CW>
+// REF: one refinement correction (FP32 factors): the forward sweep reads the
+// FP64 residual, scaled per column and rounded (in place of B), and the
+// backward sweep adds x / scale to the FP64 iterate (in place of storing X).
+template <class T, bool LOWER, int NT, int CW, bool REF>
 __global__ __launch_bounds__(NT) void sweep_kernel(int n, int nrhs, const T* LUb, int64_t strideA,
                                                    const int32_t* permb, const T* Bb, int64_t strideB, T* Xb,
-                                                   int64_t strideX, const int32_t* info, const int32_t* mask) {
+                                                   int64_t strideX, const int32_t* info, const int32_t* mask,
+                                                   RefIO ref) {
   constexpr int S = NT / GB;       // threads per row in the off-diagonal pass
   constexpr int NWV = NT / 64;     // waves
   constexpr int CPW = (CW + NWV - 1) / NWV;  // columns per wave in the diagonal solve
@@ -61,11 +73,19 @@ __global__ __launch_bounds__(NT) void sweep_kernel(int n, int nrhs, const T* LUb
   __shared__ T dg[GB][GB + 1];  // the current diagonal block (padded rows)
   constexpr int DPT = GB * GB / NT;  // diagonal-block entries per thread
 
+  // per-column scales of a refinement correction
+  double sc[CW];
+#pragma unroll
+  for (int c = 0; c < CW; ++c) sc[c] = 1.0;
+  if constexpr (REF) {
+#pragma unroll
+    for (int c = 0; c < CW; ++c) sc[c] = c < nc ? ref.scale[mat * nrhs + c0 + c] : 1.0;
+  }
   // right-hand side slice: B gathered by perm (forward) or X (backward); a
   // thread takes whole rows, RP rows per pass with their loads in flight
   {
     constexpr int RP = CW >= 16 ? 2 : 4;
-    const T* Bm = Bb + mat * strideB;
+    const T* Bm = REF ? nullptr : Bb + mat * strideB;
     const int32_t* perm = permb + mat * n;
     for (int i0 = t; i0 < n; i0 += RP * NT) {
       int src[RP];
@@ -77,9 +97,15 @@ __global__ __launch_bounds__(NT) void sweep_kernel(int n, int nrhs, const T* LUb
       T v[RP][CW];
 #pragma unroll
       for (int u = 0; u < RP; ++u) {
-        const T* br = (LOWER ? Bm : X) + (int64_t)src[u] * nrhs + c0;
+        if constexpr (REF && LOWER) {
+          const double* rr = ref.R + mat * strideB + (int64_t)src[u] * nrhs + c0;
 #pragma unroll
-        for (int c = 0; c < CW; ++c) v[u][c] = c < nc ? br[c] : T(0);
+          for (int c = 0; c < CW; ++c) v[u][c] = c < nc ? (T)(rr[c] * sc[c]) : T(0);
+        } else {
+          const T* br = (LOWER ? Bm : X) + (int64_t)src[u] * nrhs + c0;
+#pragma unroll
+          for (int c = 0; c < CW; ++c) v[u][c] = c < nc ? br[c] : T(0);
+        }
       }
 #pragma unroll
       for (int u = 0; u < RP; ++u) {
@@ -247,11 +273,23 @@ __global__ __launch_bounds__(NT) void sweep_kernel(int n, int nrhs, const T* LUb
     }
     __syncthreads();
   }
-  for (int i = t; i < n; i += NT) {
-    T* xr = X + (int64_t)i * nrhs + c0;
+  if constexpr (REF && !LOWER) {
+    for (int i = t; i < n; i += NT) {
+      double* xr = ref.X + mat * strideX + (int64_t)i * nrhs + c0;
 #pragma unroll
-    for (int c = 0; c < CW; ++c)
-      if (c < nc) xr[c] = xs[(int64_t)i * CW + c];
+      for (int c = 0; c < CW; ++c)
+        if (c < nc) {
+          const double d = (double)xs[(int64_t)i * CW + c] / sc[c];
+          xr[c] = ref.first ? d : xr[c] + d;
+        }
+    }
+  } else {
+    for (int i = t; i < n; i += NT) {
+      T* xr = X + (int64_t)i * nrhs + c0;
+#pragma unroll
+      for (int c = 0; c < CW; ++c)
+        if (c < nc) xr[c] = xs[(int64_t)i * CW + c];
+    }
   }
 }
 
@@ -314,49 +352,44 @@ int chunk_width(int64_t count, int n, int nrhs) {
 
 // Launches one sweep; dynamic LDS above the default 64 KiB is enabled once per
 // instantiation.
-template <class T, bool LOWER, int NT, int CW>
+template <class T, bool LOWER, int NT, int CW, bool REF>
 int launch_sweep_cw(dim3 grid, size_t shm, hipStream_t s, int n, int nrhs, const T* LU, int64_t strideA,
                     const int32_t* perm, const T* B, int64_t strideB, T* X, int64_t strideX, const int32_t* info,
-                    const int32_t* mask) {
-  auto* k = sweep_kernel<T, LOWER, NT, CW>;
+                    const int32_t* mask, const RefIO& ref) {
+  auto* k = sweep_kernel<T, LOWER, NT, CW, REF>;
   // static LDS (diagonal block, partials) plus the slice can pass 64 KiB:
   // opt in once per instantiation to the full dynamic budget
   static const hipError_t once = hipFuncSetAttribute(
       reinterpret_cast<const void*>(k), hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_budget());
   NPPH_HIP(once);
   (void)shm;
-  hipLaunchKernelGGL(k, grid, dim3(NT), shm, s, n, nrhs, LU, strideA, perm, B, strideB, X, strideX, info, mask);
+  hipLaunchKernelGGL(k, grid, dim3(NT), shm, s, n, nrhs, LU, strideA, perm, B, strideB, X, strideX, info, mask, ref);
   return NPPH_OK;
 }
 
-template <class T, bool LOWER, int NT>
+template <class T, bool LOWER, int NT, bool REF>
 int launch_sweep(int cw, dim3 grid, size_t shm, hipStream_t s, int n, int nrhs, const T* LU, int64_t strideA,
                  const int32_t* perm, const T* B, int64_t strideB, T* X, int64_t strideX, const int32_t* info,
-                 const int32_t* mask) {
+                 const int32_t* mask, const RefIO& ref) {
+#define NPPH_SWEEP(CWV)                                                                                         \
+  return launch_sweep_cw<T, LOWER, NT, CWV, REF>(grid, shm, s, n, nrhs, LU, strideA, perm, B, strideB, X, strideX, \
+                                                 info, mask, ref)
   if constexpr (sizeof(T) == 4) {
-    if (cw == 16)
-      return launch_sweep_cw<T, LOWER, NT, 16>(grid, shm, s, n, nrhs, LU, strideA, perm, B, strideB, X, strideX,
-                                               info, mask);
+    if (cw == 16) NPPH_SWEEP(16);
   }
-  if (cw == 8)
-    return launch_sweep_cw<T, LOWER, NT, 8>(grid, shm, s, n, nrhs, LU, strideA, perm, B, strideB, X, strideX, info,
-                                            mask);
-  if (cw == 4)
-    return launch_sweep_cw<T, LOWER, NT, 4>(grid, shm, s, n, nrhs, LU, strideA, perm, B, strideB, X, strideX, info,
-                                            mask);
-  if (cw == 2)
-    return launch_sweep_cw<T, LOWER, NT, 2>(grid, shm, s, n, nrhs, LU, strideA, perm, B, strideB, X, strideX, info,
-                                            mask);
-  return launch_sweep_cw<T, LOWER, NT, 1>(grid, shm, s, n, nrhs, LU, strideA, perm, B, strideB, X, strideX, info,
-                                          mask);
+  if (cw == 8) NPPH_SWEEP(8);
+  if (cw == 4) NPPH_SWEEP(4);
+  if (cw == 2) NPPH_SWEEP(2);
+  NPPH_SWEEP(1);
+#undef NPPH_SWEEP
 }
 
-template <class T>
-int blocked_getrs_perm(int64_t count, int n, int nrhs, const T* LU, int64_t strideA, const int32_t* perm,
-                       const T* B, int64_t strideB, T* X, int64_t strideX, const int32_t* info, const int32_t* mask,
-                       hipStream_t s) {
+template <class T, bool REF>
+int getrs_impl(int64_t count, int n, int nrhs, const T* LU, int64_t strideA, const int32_t* perm, const T* B,
+               int64_t strideB, T* X, int64_t strideX, const int32_t* info, const int32_t* mask, const RefIO& ref,
+               hipStream_t s) {
   if (count <= 0 || n <= 0 || nrhs <= 0) return NPPH_OK;
-  if (B == X) { set_error("getrs: B and X must not alias"); return NPPH_ERR_INVALID; }
+  if (!REF && B == X) { set_error("getrs: B and X must not alias"); return NPPH_ERR_INVALID; }
   if (count > 2147483647LL) { set_error("getrs: count too large"); return NPPH_ERR_INVALID; }
   const int cw = chunk_width<T>(count, n, nrhs);
   const int nz = (int)ceil_div(nrhs, cw);
@@ -366,18 +399,33 @@ int blocked_getrs_perm(int64_t count, int n, int nrhs, const T* LU, int64_t stri
   ProfScope ps(kProfGetrs, 2.0 * count * n * n * nrhs,
                (double)count * (sizeof(T) * (double)n * n + 3.0 * sizeof(T) * n * nrhs), s);
   if (n <= 256) {
-    NPPH_TRY((launch_sweep<T, true, 256>(cw, grid, shm, s, n, nrhs, LU, strideA, perm, B, strideB, X, strideX, info,
-                                         mask)));
-    NPPH_TRY((launch_sweep<T, false, 256>(cw, grid, shm, s, n, nrhs, LU, strideA, perm, B, strideB, X, strideX, info,
-                                          mask)));
+    NPPH_TRY((launch_sweep<T, true, 256, REF>(cw, grid, shm, s, n, nrhs, LU, strideA, perm, B, strideB, X, strideX,
+                                              info, mask, ref)));
+    NPPH_TRY((launch_sweep<T, false, 256, REF>(cw, grid, shm, s, n, nrhs, LU, strideA, perm, B, strideB, X, strideX,
+                                               info, mask, ref)));
   } else {
-    NPPH_TRY((launch_sweep<T, true, 512>(cw, grid, shm, s, n, nrhs, LU, strideA, perm, B, strideB, X, strideX, info,
-                                         mask)));
-    NPPH_TRY((launch_sweep<T, false, 512>(cw, grid, shm, s, n, nrhs, LU, strideA, perm, B, strideB, X, strideX, info,
-                                          mask)));
+    NPPH_TRY((launch_sweep<T, true, 512, REF>(cw, grid, shm, s, n, nrhs, LU, strideA, perm, B, strideB, X, strideX,
+                                              info, mask, ref)));
+    NPPH_TRY((launch_sweep<T, false, 512, REF>(cw, grid, shm, s, n, nrhs, LU, strideA, perm, B, strideB, X, strideX,
+                                               info, mask, ref)));
   }
   NPPH_HIP(hipGetLastError());
   return NPPH_OK;
+}
+
+template <class T>
+int blocked_getrs_perm(int64_t count, int n, int nrhs, const T* LU, int64_t strideA, const int32_t* perm,
+                       const T* B, int64_t strideB, T* X, int64_t strideX, const int32_t* info, const int32_t* mask,
+                       hipStream_t s) {
+  return getrs_impl<T, false>(count, n, nrhs, LU, strideA, perm, B, strideB, X, strideX, info, mask, RefIO{}, s);
+}
+
+int getrs_refine(int64_t count, int n, int nrhs, const float* LU, int64_t strideA, const int32_t* perm,
+                 const double* R, const double* scale, float* Y, double* X, int first, const int32_t* mask,
+                 hipStream_t s) {
+  const int64_t per = (int64_t)n * nrhs;
+  return getrs_impl<float, true>(count, n, nrhs, LU, strideA, perm, nullptr, per, Y, per, nullptr, mask,
+                                 RefIO{R, scale, X, first}, s);
 }
 
 template int blocked_getrs_perm<double>(int64_t, int, int, const double*, int64_t, const int32_t*, const double*,

@@ -132,31 +132,6 @@ __device__ __forceinline__ double pow2_scale(double m) {
   return ldexp(1.0, -ilogb(m));
 }
 
-// Rlo = fp32(R * scale) (per column scale), only for active systems.
-__global__ void scale_demote_kernel(int64_t count, int n, int nrhs, const double* R, const double* scale,
-                                    float* Rlo, const int32_t* active) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t per = (int64_t)n * nrhs;
-  if (i >= count * per) return;
-  const int64_t mat = i / per;
-  if (active[mat] != 1) return;
-  const int c = (int)(i % nrhs);
-  Rlo[i] = (float)(R[i] * scale[mat * nrhs + c]);
-}
-
-// X (+)= D / scale for active systems; first sweep writes instead of adding.
-__global__ void update_kernel(int64_t count, int n, int nrhs, double* X, const float* D, const double* scale,
-                              const int32_t* active, int first) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t per = (int64_t)n * nrhs;
-  if (i >= count * per) return;
-  const int64_t mat = i / per;
-  if (active[mat] != 1) return;
-  const int c = (int)(i % nrhs);
-  const double d = (double)D[i] / scale[mat * nrhs + c];
-  X[i] = first ? d : X[i] + d;
-}
-
 __global__ void count_sweep_kernel(int64_t count, const int32_t* active, int32_t* iters) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < count && active[i] == 1) iters[i] += 1;
@@ -686,18 +661,14 @@ int refine_solve(const npph_refine_opts& o, const double* A, const RefineFactors
     hipLaunchKernelGGL(init_state_kernel, g1(count), dim3(256), 0, s, count, info, active, iters, nullptr);
     NPPH_TRY(report());
   } else {
-    float *Rlo, *Dlo;
-    NPPH_TRY(workspace(kWsRlo, sizeof(float) * (size_t)nb_all, (void**)&Rlo));
+    float* Dlo;
     NPPH_TRY(workspace(kWsD, sizeof(float) * (size_t)nb_all, (void**)&Dlo));
     FlagSlot* fs;
     NPPH_TRY(flag_slot(&fs));
     hipLaunchKernelGGL(init_state_kernel, g1(count), dim3(256), 0, s, count, info, active, iters, ctr + 1);
     hipLaunchKernelGGL(scale_from_bmax_kernel, g1(nk), dim3(256), 0, s, nk, bmax, scale);
     // Sweep 0 (x = 0, r = b).
-    hipLaunchKernelGGL(scale_demote_kernel, g1(nb_all), dim3(256), 0, s, count, n, nrhs, B, scale, Rlo, active);
-    NPPH_TRY(blocked_getrs_perm<float>(count, n, nrhs, f.lu32, per_a, f.perm, Rlo, per_b, Dlo, per_b, nullptr, active,
-                                       s));
-    hipLaunchKernelGGL(update_kernel, g1(nb_all), dim3(256), 0, s, count, n, nrhs, X, Dlo, scale, active, 1);
+    NPPH_TRY(getrs_refine(count, n, nrhs, f.lu32, per_a, f.perm, B, scale, Dlo, X, 1, active, s));
     hipLaunchKernelGGL(count_sweep_kernel, g1(count), dim3(256), 0, s, count, active, iters);
     NPPH_HIP(hipGetLastError());
     const int max_iter = o.max_iter > 0 ? o.max_iter : 10;
@@ -717,10 +688,7 @@ int refine_solve(const npph_refine_opts& o, const double* A, const RefineFactors
       // zero by then; the bound only guards the loop
       const bool more = it <= max_iter;
       if (more) {
-        hipLaunchKernelGGL(scale_demote_kernel, g1(nb_all), dim3(256), 0, s, count, n, nrhs, R, scale, Rlo, active);
-        NPPH_TRY(blocked_getrs_perm<float>(count, n, nrhs, f.lu32, per_a, f.perm, Rlo, per_b, Dlo, per_b, nullptr,
-                                           active, s));
-        hipLaunchKernelGGL(update_kernel, g1(nb_all), dim3(256), 0, s, count, n, nrhs, X, Dlo, scale, active, 0);
+        NPPH_TRY(getrs_refine(count, n, nrhs, f.lu32, per_a, f.perm, R, scale, Dlo, X, 0, active, s));
         hipLaunchKernelGGL(count_sweep_kernel, g1(count), dim3(256), 0, s, count, active, iters);
         NPPH_HIP(hipGetLastError());
       }
