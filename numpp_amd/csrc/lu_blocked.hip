@@ -569,21 +569,20 @@ struct Rec {
     // no rows, and the per-column barrier spans fewer waves); taller panels
     // measured faster with 512 threads and fewer rows per thread
     // Many matrices (at least four per CU): one or two waves per matrix with
-    // up to two rows per lane; the per-column pivot search, pivot-row exchange
-    // and barrier then cost one or two waves' instructions instead of four,
-    // and the other matrices' waves hide the column chain's latency.
+    // one row per lane up to 128 rows and two up to 256; the per-column pivot
+    // search, pivot-row exchange and barrier then cost one or two waves'
+    // instructions instead of four, and the other matrices' waves hide the
+    // column chain's latency.
     static const int thin_env = [] {
       const char* e = std::getenv("NUMPP_HIP_LEAF_THIN");
       return e ? std::atoi(e) : 1;
     }();
     const bool thin = thin_env != 0 && count >= 4 * (int64_t)std::max(cu_count(), 1);
-    // FP64 rows take twice the registers: above 64 rows it keeps one row per
-    // thread up to 128 rows and 16-column inner blocks up to 256 (measured).
+    // (FP64 rows take twice the registers: between 128 and 256 rows it uses
+    // 16-column inner blocks; measured.)
     if (thin && m <= 64) launch_leaf<T, 1, 64, 64>(grid, shm, la);
-    else if (thin && m <= 128) {
-      if constexpr (F) launch_leaf<T, 2, 64, 64>(grid, shm, la);
-      else launch_leaf<T, 1, 64, 128>(grid, shm, la);
-    } else if (thin && m <= 256) {
+    else if (thin && m <= 128) launch_leaf<T, 1, 64, 128>(grid, shm, la);
+    else if (thin && m <= 256) {
       if constexpr (F) launch_leaf<T, 2, 64, 128>(grid, shm, la);
       else launch_leaf<T, 2, 16, 128>(grid, shm, la);
     } else if (m <= PT / 2) launch_leaf<T, 1, 64, PT / 2>(grid, shm, la);
