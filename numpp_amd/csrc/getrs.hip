@@ -121,18 +121,25 @@ __global__ __launch_bounds__(NT) void sweep_kernel(int n, int nrhs, const T* LUb
 
   const int nblk = (n + GB - 1) / GB;
   const int r = t / S, sg = t - r * S;
+  // A step's diagonal block, loaded by all threads (coalesced, clamped
+  // indices): the first before the loop, each next one as soon as the current
+  // one is in LDS, so its latency hides behind the diagonal solve and the next
+  // off-diagonal pass.
+  T dpre[DPT];
+  auto load_diag = [&](int st) {
+    const int b = LOWER ? st : nblk - 1 - st;
+    const int q0 = b * GB, qs = min(GB, n - q0);
+#pragma unroll
+    for (int u = 0; u < DPT; ++u) {
+      const int e = t + u * NT, rr = e / GB, cc = e - rr * GB;
+      dpre[u] = LU[(int64_t)(q0 + min(rr, qs - 1)) * n + q0 + min(cc, qs - 1)];
+    }
+  };
+  load_diag(0);
   for (int step = 0; step < nblk; ++step) {
     const int bi = LOWER ? step : nblk - 1 - step;
     const int r0 = bi * GB, bs = min(GB, n - r0);
     const int klo = LOWER ? 0 : r0 + bs, khi = LOWER ? r0 : n;
-    // the diagonal block, loaded by all threads (coalesced, clamped indices)
-    // before the off-diagonal pass so its latency hides behind it
-    T dpre[DPT];
-#pragma unroll
-    for (int u = 0; u < DPT; ++u) {
-      const int e = t + u * NT, rr = e / GB, cc = e - rr * GB;
-      dpre[u] = LU[(int64_t)(r0 + min(rr, bs - 1)) * n + r0 + min(cc, bs - 1)];
-    }
     if constexpr (NT == 256) {
       // ---- off-diagonal contributions as a (64 x K) x (K x 16) product on
       // MFMA 16x16x4: wave w takes block rows 16 w .. 16 w + 15; a lane loads
@@ -227,6 +234,7 @@ __global__ __launch_bounds__(NT) void sweep_kernel(int n, int nrhs, const T* LUb
       const int e = t + u * NT, rr = e / GB, cc = e - rr * GB;
       dg[rr][cc] = (rr < bs && cc < bs) ? dpre[u] : T(0);
     }
+    if (step + 1 < nblk) load_diag(step + 1);
     __syncthreads();
     // ---- diagonal block: wave w solves columns w, w + NWV, ... (lane = row)
     if (wave < CW) {
