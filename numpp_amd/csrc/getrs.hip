@@ -245,14 +245,25 @@ __global__ __launch_bounds__(NT) void sweep_kernel(int n, int nrhs, const T* LUb
         const int c = wave + j * NWV;
         x[j] = (lane < bs && c < CW) ? xs[(int64_t)i * CW + c] - part[lane][c] : T(0);
       }
-      // steps q >= bs touch only padding rows: skipped (bs is uniform)
+      // The lane's factor entries come from LDS DCH steps at a time, all
+      // loads of a group issued together (one LDS round trip per group, not
+      // per step); groups past the block's bs rows are skipped (bs is
+      // uniform) and the steps of a group past them are masked to no-ops.
+      constexpr int DCH = sizeof(T) == 4 ? 8 : 4;
       if (LOWER) {
 #pragma unroll
-        for (int q = 0; q < GB; ++q) {
-          if (q + 1 >= bs) break;
-          const T f = lane > q ? dg[lane][q] : T(0);
+        for (int q0 = 0; q0 < GB; q0 += DCH) {
+          if (q0 + 1 >= bs) break;
+          T fv[DCH];
 #pragma unroll
-          for (int j = 0; j < CPW; ++j) x[j] = x[j] - f * readlane<T>(x[j], q);
+          for (int u = 0; u < DCH; ++u) fv[u] = dg[lane][q0 + u];
+#pragma unroll
+          for (int u = 0; u < DCH; ++u) {
+            const int q = q0 + u;
+            const T f = (lane > q && q + 1 < bs) ? fv[u] : T(0);
+#pragma unroll
+            for (int j = 0; j < CPW; ++j) x[j] = x[j] - f * readlane<T>(x[j], q);
+          }
         }
       } else {
         const T d = lane < bs ? dg[lane][lane] : T(1);
@@ -260,15 +271,23 @@ __global__ __launch_bounds__(NT) void sweep_kernel(int n, int nrhs, const T* LUb
         // then a multiply (within one rounding of lu_solve's division)
         const T dinv = T(1) / d;
 #pragma unroll
-        for (int q = GB - 1; q >= 0; --q) {
-          if (q >= bs) continue;
-          if (lane == q) {
+        for (int q0 = GB - DCH; q0 >= 0; q0 -= DCH) {
+          if (q0 >= bs) continue;
+          T fv[DCH];
 #pragma unroll
-            for (int j = 0; j < CPW; ++j) x[j] = x[j] * dinv;
+          for (int u = 0; u < DCH; ++u) fv[u] = dg[lane][q0 + u];
+#pragma unroll
+          for (int u = DCH - 1; u >= 0; --u) {
+            const int q = q0 + u;
+            const bool live = q < bs;
+            if (lane == q && live) {
+#pragma unroll
+              for (int j = 0; j < CPW; ++j) x[j] = x[j] * dinv;
+            }
+            const T f = (lane < q && live) ? fv[u] : T(0);
+#pragma unroll
+            for (int j = 0; j < CPW; ++j) x[j] = x[j] - f * readlane<T>(x[j], q);
           }
-          const T f = lane < q ? dg[lane][q] : T(0);
-#pragma unroll
-          for (int j = 0; j < CPW; ++j) x[j] = x[j] - f * readlane<T>(x[j], q);
         }
       }
       if (lane < bs) {
