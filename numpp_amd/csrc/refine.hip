@@ -13,7 +13,6 @@
 #include <vector>
 
 #include "common.hpp"
-#include "gemm.hpp"
 #include "lu.hpp"
 #include "prof.hpp"
 
@@ -368,45 +367,6 @@ __global__ __launch_bounds__(256) void residual_small_kernel(int n, int nrhs, co
   }
 }
 
-// Column maxima of R and X for small orders: one workgroup per system, 16
-// columns at a time, 16 row groups reduced through LDS. The residual itself
-// (R = B - A X) comes from the batched MFMA GEMM (alpha = -1, beta = 1 on a
-// copy of B), which reads A once at full width.
-constexpr int RS_N = 256;
-__global__ __launch_bounds__(256) void colmax_kernel(int n, int nrhs, const double* Rb, const double* Xb,
-                                                     double* rmax, double* xmax, const int32_t* active) {
-  const int64_t mat = blockIdx.x;
-  if (active && active[mat] != 1) return;
-  const int64_t per = (int64_t)n * nrhs;
-  const double* __restrict__ R = Rb + mat * per;
-  const double* __restrict__ X = Xb + mat * per;
-  const int t = threadIdx.x, c = t & 15, g = t >> 4;
-  __shared__ double red_r[16][16], red_x[16][16];
-  for (int c0 = 0; c0 < nrhs; c0 += 16) {
-    double rm = 0.0, xm = 0.0;
-    if (c0 + c < nrhs) {
-      for (int i = g; i < n; i += 16) {
-        rm = fmax(rm, fabs(R[(int64_t)i * nrhs + c0 + c]));
-        xm = fmax(xm, fabs(X[(int64_t)i * nrhs + c0 + c]));
-      }
-    }
-    red_r[g][c] = rm;
-    red_x[g][c] = xm;
-    __syncthreads();
-    if (t < 16 && c0 + t < nrhs) {
-      double r = 0.0, x = 0.0;
-#pragma unroll
-      for (int q = 0; q < 16; ++q) {
-        r = fmax(r, red_r[q][t]);
-        x = fmax(x, red_x[q][t]);
-      }
-      rmax[mat * nrhs + c0 + t] = r;
-      xmax[mat * nrhs + c0 + t] = x;
-    }
-    __syncthreads();
-  }
-}
-
 // Convergence test per system (one thread per system). Records eta per column,
 // decides active (1 = continue, 0 = converged, 2 = stopped without
 // convergence), computes the next sweep's scales, clears the max buffers and
@@ -501,16 +461,6 @@ int residual_launch(int64_t count, int n, int nrhs, const double* A, const doubl
     else
       hipLaunchKernelGGL(residual_small_kernel<false>, grid, dim3(256), 0, s, n, nrhs, A, X, B, R, rmax, xmax,
                          active);
-    NPPH_HIP(hipGetLastError());
-    return NPPH_OK;
-  }
-  if (n <= RS_N && count <= 65535) {
-    ProfScope ps(kProfResidual, 2.0 * count * n * n * nrhs, 8.0 * count * ((double)n * n + 3.0 * n * nrhs), s);
-    NPPH_HIP(hipMemcpyAsync(R, B, sizeof(double) * (size_t)count * n * nrhs, hipMemcpyDeviceToDevice, s));
-    const int64_t per_b = (int64_t)n * nrhs;
-    GemmArgs<double> g{n, nrhs, n, -1.0, 1.0, A, n, (int64_t)n * n, X, nrhs, per_b, R, nrhs, per_b, 0, 0, 0};
-    NPPH_TRY(gemm_launch<double>(g, count, s, -2));
-    hipLaunchKernelGGL(colmax_kernel, dim3((unsigned)count), dim3(256), 0, s, n, nrhs, R, X, rmax, xmax, active);
     NPPH_HIP(hipGetLastError());
     return NPPH_OK;
   }
