@@ -17,6 +17,8 @@
 // XCD (workgroups are dispatched round-robin over the 8 XCDs, each with its own
 // L2), then rastered in groups of 8 tile rows so an XCD's tiles share A row
 // panels and B column panels in its L2.
+#include <algorithm>
+
 #include "common.hpp"
 #include "gemm.hpp"
 #include "prof.hpp"
@@ -211,8 +213,17 @@ int gemm_launch(GemmArgs<T> p, int64_t batch, hipStream_t s, int prof) {
   }
   // 64-wide tiles for narrow (N <= 64) or short (M <= 64) products (the
   // lowest recursion levels), where a 128-wide tile would leave half of its
-  // MFMA columns or rows empty
-  const bool narrow = p.N <= 64, shortm = p.M <= 64;
+  // MFMA columns or rows empty; and for batches too small to give every CU
+  // two 128 x 128 tiles (the recursion's middle levels on few matrices),
+  // where narrower tiles put more CUs to work
+  bool narrow = p.N <= 64, shortm = p.M <= 64;
+  const int64_t cus = std::max(cu_count(), 1);
+  auto wgs = [&](int bm, int bn) { return ceil_div(p.M, bm) * ceil_div(p.N, bn) * batch; };
+  if (wgs(shortm ? 64 : 128, narrow ? 64 : 128) < 2 * cus) {
+    if (!narrow && p.N >= p.M) narrow = true;
+    else if (!shortm) shortm = true;
+    if (wgs(shortm ? 64 : 128, narrow ? 64 : 128) < 2 * cus) narrow = shortm = true;
+  }
   const int BN = narrow ? 64 : 128, BM = shortm ? 64 : 128;
   p.tiles_m = (int)ceil_div(p.M, BM);
   p.tiles_n = (int)ceil_div(p.N, BN);
