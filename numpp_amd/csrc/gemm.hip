@@ -213,16 +213,17 @@ int gemm_launch(GemmArgs<T> p, int64_t batch, hipStream_t s, int prof) {
   }
   // 64-wide tiles for narrow (N <= 64) or short (M <= 64) products (the
   // lowest recursion levels), where a 128-wide tile would leave half of its
-  // MFMA columns or rows empty; and for batches too small to give every CU
-  // two 128 x 128 tiles (the recursion's middle levels on few matrices),
-  // where narrower tiles put more CUs to work
+  // MFMA columns or rows empty; and for products of at most 2048 x 1024
+  // outputs whose batch gives fewer than 16 128 x 128 tiles per CU (the
+  // recursion's middle levels), where narrower tiles put more CUs to work
+  // (measured)
   bool narrow = p.N <= 64, shortm = p.M <= 64;
   const int64_t cus = std::max(cu_count(), 1);
   auto wgs = [&](int bm, int bn) { return ceil_div(p.M, bm) * ceil_div(p.N, bn) * batch; };
-  if (wgs(shortm ? 64 : 128, narrow ? 64 : 128) < 2 * cus) {
+  if (p.M * p.N <= 2048 * 1024 && wgs(shortm ? 64 : 128, narrow ? 64 : 128) < 16 * cus) {
     if (!narrow && p.N >= p.M) narrow = true;
     else if (!shortm) shortm = true;
-    if (wgs(shortm ? 64 : 128, narrow ? 64 : 128) < 2 * cus) narrow = shortm = true;
+    if (wgs(shortm ? 64 : 128, narrow ? 64 : 128) < 16 * cus) narrow = shortm = true;
   }
   const int BN = narrow ? 64 : 128, BM = shortm ? 64 : 128;
   p.tiles_m = (int)ceil_div(p.M, BM);
