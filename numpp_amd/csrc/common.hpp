@@ -35,7 +35,7 @@ int hip_fail(hipError_t e, const char* what, const char* file, int line);
 
 // ---- device state ----
 // One process-wide context per device, created once by ensure_init(): device
-// probe (gfx950 check), a default stream, and a grow-only workspace arena.
+// probe (gfx950 check) and a default stream for host-pointer calls.
 struct Workspace {
   void* ptr = nullptr;
   size_t bytes = 0;
@@ -44,31 +44,66 @@ struct Workspace {
 int ensure_init();                // NPPH_OK or NPPH_ERR_NO_DEVICE
 hipStream_t default_stream();     // library-owned stream for host-pointer calls
 int cu_count();                   // compute units on the device
-// Returns a device buffer of at least `bytes` from a named workspace slot
-// (grow-only, freed at process exit). Slots let concurrent phases of one call
-// hold separate buffers; callers that share a slot must not overlap.
+enum WsSlot { kWsA = 0, kWsB, kWsPiv, kWsInfo, kWsLo, kWsR, kWsX, kWsNorm, kWsMisc, kWsMove, kWsPerm, kWsD, kWsNet, kWsFlags, kWsPart, kWsCount };
+
+// Scratch memory. Every entry point runs inside a CallScope for the stream it
+// issues on. The scope binds that stream's workspace set to the calling thread
+// and holds the set's lock until the entry point returns, so
+//   - work issued on one stream reuses that stream's buffers in stream order,
+//   - calls on different streams never share a buffer,
+//   - two threads issuing on the same stream take turns (a host-pointer call
+//     holds the lock through its final synchronisation),
+//   - growing a slot waits only for the set's own stream before freeing.
+// workspace() returns a device buffer of at least `bytes` from a named slot of
+// the bound set (grow-only; freed when the stream is destroyed through
+// npph_stream_destroy, else at process exit).
 int workspace(int slot, size_t bytes, void** out);
 // Incremented whenever a workspace slot is reallocated; a captured hipGraph
 // that baked in workspace addresses is stale once this changes.
 uint64_t workspace_generation();
 // True while `s` is being captured into a hipGraph (no allocation, no sync).
 bool stream_capturing(hipStream_t s);
-enum WsSlot { kWsA = 0, kWsB, kWsPiv, kWsInfo, kWsLo, kWsR, kWsX, kWsNorm, kWsMisc, kWsMove, kWsPerm, kWsD, kWsNet, kWsFlags, kWsPart, kWsCount };
 
-// A private set of workspace slots. A thread that binds one (ws_bind) gets its
-// own buffers from workspace(), so calls running concurrently on different
-// streams from different threads never share scratch memory; growth of a
-// bound set waits only for the set's owner stream.
 struct WsSet;
+// A private set (a serving worker's): bound with ws_bind for the worker's
+// lifetime; CallScopes on that thread then leave it in place.
 WsSet* ws_create(hipStream_t owner);
 void ws_destroy(WsSet* w);
-void ws_bind(WsSet* w);  // nullptr = the process-wide set
-WsSet* ws_current();     // the calling thread's bound set, or nullptr
+void ws_bind(WsSet* w);
+bool ws_private();  // the calling thread has a private set bound
+// Frees the per-stream set of `s` (after synchronising s).
+void ws_release_stream(hipStream_t s);
+
+class CallScope {
+ public:
+  explicit CallScope(hipStream_t s);
+  ~CallScope();
+  CallScope(const CallScope&) = delete;
+  CallScope& operator=(const CallScope&) = delete;
+
+ private:
+  WsSet* set_ = nullptr;
+};
+
+// Caller-provided scratch in place of the workspace slots (the allocation-free
+// entry point): while a ScratchArena is bound on a thread, workspace() carves
+// 256-byte-aligned regions out of [base, base + bytes) instead of allocating
+// (a slot asked for again gets its earlier region when that is large enough),
+// and fails with NPPH_ERR_INVALID when the arena is full.
+struct ScratchArena {
+  char* base = nullptr;
+  size_t bytes = 0;
+  size_t used = 0;
+  size_t off[kWsCount] = {};
+  size_t len[kWsCount] = {};
+};
+void arena_bind(ScratchArena* a);  // nullptr unbinds
+bool arena_bound();
 
 // While a PlainIssue is alive on a thread, large factorisations on that thread
-// are issued as plain launches on the caller's stream (no sub-batch streams, no
-// cached graphs): a caller that runs work eagerly and then captures the same
-// work needs both runs to take the same path and touch the same workspaces.
+// are issued as plain launches on the caller's stream (no cached graphs): a
+// caller that runs work eagerly and then captures the same work needs both
+// runs to take the same path and touch the same workspaces.
 struct PlainIssue {
   PlainIssue();
   ~PlainIssue();
@@ -107,6 +142,16 @@ __device__ __forceinline__ bool piv_better(double v1, int p1, double v0, int p0)
 }
 __device__ __forceinline__ bool piv_better(float v1, int p1, float v0, int p0) {
   return v1 > v0 || (v1 == v0 && p1 < p0);
+}
+
+// Pivot-search key of entry a at current position pos for step k, matching
+// NumPP's search (linalg.cpp:55-57): it starts from row k and moves to a later
+// row only when that row's |a| is strictly larger. A NaN at row k therefore is
+// never replaced (key +inf, and on a tie the smallest position wins), and a NaN
+// below it is never chosen (key -1, below every real candidate).
+template <class V>
+__device__ __forceinline__ V piv_key(V a, int pos, int k) {
+  return a != a ? (pos == k ? V(INFINITY) : V(-1)) : fabs(a);
 }
 
 // One exchange step of the argmax: each lane compares with the lane that the

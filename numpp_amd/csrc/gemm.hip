@@ -256,16 +256,20 @@ int npph_dgemm_dev(int64_t m, int64_t n, int64_t k, double alpha, const double* 
                    int64_t sA, const double* B, int64_t ldb, int64_t sB, double beta, double* C,
                    int64_t ldc, int64_t sC, int64_t batch, void* stream) {
   NPPH_TRY(ensure_init());
+  const hipStream_t s = stream ? (hipStream_t)stream : default_stream();
+  CallScope cs(s);
   GemmArgs<double> p{m, n, k, alpha, beta, A, lda, sA, B, ldb, sB, C, ldc, sC, 0, 0, 0};
-  return gemm_launch<double>(p, batch, stream ? (hipStream_t)stream : default_stream());
+  return gemm_launch<double>(p, batch, s);
 }
 
 int npph_sgemm_dev(int64_t m, int64_t n, int64_t k, float alpha, const float* A, int64_t lda,
                    int64_t sA, const float* B, int64_t ldb, int64_t sB, float beta, float* C,
                    int64_t ldc, int64_t sC, int64_t batch, void* stream) {
   NPPH_TRY(ensure_init());
+  const hipStream_t s = stream ? (hipStream_t)stream : default_stream();
+  CallScope cs(s);
   GemmArgs<float> p{m, n, k, alpha, beta, A, lda, sA, B, ldb, sB, C, ldc, sC, 0, 0, 0};
-  return gemm_launch<float>(p, batch, stream ? (hipStream_t)stream : default_stream());
+  return gemm_launch<float>(p, batch, s);
 }
 
 int npph_gemm(int dtype, int64_t m, int64_t n, int64_t k, const void* A, const void* B, void* C) {
@@ -283,6 +287,9 @@ int npph_gemm(int dtype, int64_t m, int64_t n, int64_t k, const void* A, const v
     std::memset(C, 0, cb);
     return NPPH_OK;
   }
+  // the call scope is held through the final synchronisation, so concurrent
+  // host-pointer calls take turns on the library stream and its buffers
+  CallScope cs(s);
   void *dA, *dB, *dC;
   NPPH_TRY(workspace(kWsA, ab, &dA));
   NPPH_TRY(workspace(kWsB, bb, &dB));

@@ -78,8 +78,10 @@ __global__ __launch_bounds__(NTH) void bf16split_kernel(int64_t M, int64_t N, in
     amax = fmaxf(fmaxf(red[0][0], red[0][1]), fmaxf(red[0][2], red[0][3]));
     bmax = fmaxf(fmaxf(red[1][0], red[1][1]), fmaxf(red[1][2], red[1][3]));
     // Power-of-two scales putting each chunk panel's largest magnitude in [1, 2).
-    const int ea = amax > 0.f ? ilogbf(amax) : 0;
-    const int eb = bmax > 0.f ? ilogbf(bmax) : 0;
+    // (a zero, Inf or NaN panel is not scaled, so Inf and NaN reach C as they
+    // are; exponents are clamped to [-126, 127] so that both scales are finite)
+    const int ea = (amax > 0.f && isfinite(amax)) ? max(ilogbf(amax), -126) : 0;
+    const int eb = (bmax > 0.f && isfinite(bmax)) ? max(ilogbf(bmax), -126) : 0;
     const float sa = ldexpf(1.f, -ea), sb = ldexpf(1.f, -eb);
     const float unscale = ldexpf(1.f, ea + eb);
 #pragma unroll
@@ -192,6 +194,7 @@ extern "C" int npph_bf16split_update_dev(int64_t M, int64_t N, int64_t K, const 
                                          const float* B, int64_t ldb, float* C, int64_t ldc, int terms,
                                          void* stream) {
   NPPH_TRY(npph::ensure_init());
-  return npph::gemm_bf16split_update(M, N, K, A, lda, 0, B, ldb, 0, C, ldc, 0, 1, terms,
-                                     stream ? (hipStream_t)stream : npph::default_stream());
+  const hipStream_t s = stream ? (hipStream_t)stream : npph::default_stream();
+  npph::CallScope cs(s);
+  return npph::gemm_bf16split_update(M, N, K, A, lda, 0, B, ldb, 0, C, ldc, 0, 1, terms, s);
 }

@@ -62,7 +62,9 @@ __global__ __launch_bounds__(NT) void sweep_kernel(int n, int nrhs, const T* LUb
   // than deeper batches (measured)
   constexpr int M = NT > 256 ? (sizeof(T) == 8 ? 16 : 32) : 8;
   const int64_t mat = blockIdx.x;
-  if ((info && info[mat] != 0) || (mask && mask[mat] == 0)) return;
+  // a refinement correction touches only systems still iterating (state 1);
+  // converged (0) and stopped (2) systems keep their iterate
+  if ((info && info[mat] != 0) || (mask && (REF ? mask[mat] != 1 : mask[mat] == 0))) return;
   const int c0 = (int)blockIdx.y * CW, nc = min(CW, nrhs - c0);
   const T* __restrict__ LU = LUb + mat * strideA;
   T* __restrict__ X = Xb + mat * strideX;

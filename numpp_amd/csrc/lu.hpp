@@ -62,8 +62,8 @@ int blocked_getrs_perm(int64_t count, int n, int nrhs, const T* LU, int64_t stri
 
 // One refinement correction with FP32 factors: D = U^-1 L^-1 P fp32(R * scale)
 // (per-column scale), then X = D / scale (first) or X += D / scale, in two
-// sweeps; Y (count x n x nrhs FP32) holds the forward sweep's result. Systems
-// with mask == 0 are skipped.
+// sweeps; Y (count x n x nrhs FP32) holds the forward sweep's result. Only
+// systems with mask == 1 (still iterating) are corrected.
 int getrs_refine(int64_t count, int n, int nrhs, const float* LU, int64_t strideA, const int32_t* perm,
                  const double* R, const double* scale, float* Y, double* X, int first, const int32_t* mask,
                  hipStream_t s);

@@ -56,7 +56,9 @@ int npph_getrf_batched_dev(int64_t count, int64_t n, double* A, int64_t strideA,
     set_error("getrf: bad sizes (count=%ld n=%ld stride=%ld)", (long)count, (long)n, (long)strideA);
     return NPPH_ERR_INVALID;
   }
-  return gesv_dev_impl(count, (int)n, 0, A, strideA, nullptr, 0, ipiv, info, pick(stream));
+  const hipStream_t s = pick(stream);
+  CallScope cs(s);
+  return gesv_dev_impl(count, (int)n, 0, A, strideA, nullptr, 0, ipiv, info, s);
 }
 
 int npph_getrs_batched_dev(int64_t count, int64_t n, int64_t nrhs, const double* LU, int64_t strideA,
@@ -68,6 +70,7 @@ int npph_getrs_batched_dev(int64_t count, int64_t n, int64_t nrhs, const double*
   }
   if (count == 0 || n == 0 || nrhs == 0) return NPPH_OK;
   hipStream_t s = pick(stream);
+  CallScope cs(s);
   int32_t* perm;
   double* X;
   const int64_t sx = n * nrhs;
@@ -92,7 +95,9 @@ int npph_gesv_batched_dev(int64_t count, int64_t n, int64_t nrhs, double* A, dou
     set_error("gesv: bad sizes (count=%ld n=%ld nrhs=%ld)", (long)count, (long)n, (long)nrhs);
     return NPPH_ERR_INVALID;
   }
-  return gesv_dev_impl(count, (int)n, (int)nrhs, A, n * n, B, n * nrhs, ipiv, info, pick(stream));
+  const hipStream_t s = pick(stream);
+  CallScope cs(s);
+  return gesv_dev_impl(count, (int)n, (int)nrhs, A, n * n, B, n * nrhs, ipiv, info, s);
 }
 
 int npph_gesv_batched(int dtype, int64_t count, int64_t n, int64_t nrhs, void* A, void* B, int32_t* ipiv,
@@ -123,6 +128,8 @@ int npph_gesv_batched(int dtype, int64_t count, int64_t n, int64_t nrhs, void* A
     srcA = ha.data();
     srcB = hb.data();
   }
+  // held through the final synchronisation (see npph_gemm)
+  CallScope cs(s);
   double *dA, *dB;
   int32_t *dp, *di;
   NPPH_TRY(workspace(kWsA, sizeof(double) * na, (void**)&dA));

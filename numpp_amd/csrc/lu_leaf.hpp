@@ -1,0 +1,357 @@
+// Panel leaf of the recursive LU (see lu_blocked.hip): the kernel template and
+// its launch helpers. The instantiations are spread over the lu_leaf_*.hip
+// translation units (one per working precision and panel family) so that they
+// compile in parallel.
+//
+// The leaf factors an (n-k0) x w panel (w <= 64) with one workgroup per
+// matrix, left-looking in inner blocks of IB columns: the block's columns are
+// gathered through the leaf's row permutation (rows are not moved until the
+// block is done), brought up to date with the block's U rows (one wave per
+// column solves with L11 via v_readlane) and the L columns already computed,
+// then factored in registers with one workgroup barrier per column: each wave
+// offers its local pivot candidate together with that row's values, so every
+// thread picks the pivot (NumPP's rule: largest |a|, then smallest current
+// position, linalg.cpp:55-56, with its NaN behaviour) and reads the pivot row in
+// the same step. The per-step pivot rows (ipiv), the permutation (perm), the
+// first exact-zero pivot (info) and the leaf's net row moves for the other
+// columns are written. The rank-1 updates of linalg.cpp:64-68 are regrouped into
+// these blocks; each entry receives the same terms in a different association
+// order.
+#pragma once
+
+#include <utility>
+
+#include "common.hpp"
+
+namespace npph {
+
+constexpr int LEAF = 64;   // leaf panel width
+constexpr int PT = 512;    // leaf threads
+constexpr int MVLEAF = 2 * LEAF;  // max net moves of one leaf
+constexpr int LSTRIDE = 1 + 2 * MVLEAF;
+
+template <class T>
+struct LeafArgs {
+  T* A;
+  int64_t strideA;
+  int n, k0, w;
+  int32_t* ipiv;
+  int32_t* perm;
+  int32_t* info;
+  int32_t* mv;        // this leaf's list for matrix 0; matrix i at mv + i * mv_stride
+  int64_t mv_stride;
+  const int32_t* mask;
+};
+
+// NTH threads (512, or 256 for panels of at most 256 rows).
+template <class T, int R, int IB, bool FULL, int NTH>
+__global__ __launch_bounds__(NTH) void leaf_kernel(LeafArgs<T> a) {
+  constexpr int NWL = NTH / 64;
+  const int64_t mat = blockIdx.x;
+  if (a.mask && a.mask[mat] == 0) return;
+  T* __restrict__ A = a.A + mat * a.strideA;
+  const int n = a.n, k0 = a.k0, w = a.w;
+  const int m = n - k0;
+  int32_t* ipiv = a.ipiv + mat * n;
+  int32_t* perm = a.perm + mat * n;
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  constexpr int QC = (R >= 4 && sizeof(T) == 8) ? 4 : 8;  // L columns per update chunk
+  static_assert(IB % QC == 0 && LEAF % IB == 0, "inner blocks must hold whole update chunks");
+
+  // rows/columns left of a block: at most LEAF - IB (none when IB = LEAF)
+  constexpr int LEFT = LEAF - IB > 0 ? LEAF - IB : 1;
+  // L rows of the blocks before the last, for the later blocks' U solves
+  __shared__ T L11[LEFT][LEFT + 1];
+  __shared__ T Ublk[LEFT][IB];
+  __shared__ T red_v[2][NWL];  // candidate magnitudes (exact in T)
+  __shared__ int red_p[2][NWL];
+  __shared__ T red_row[2][NWL][IB];
+  __shared__ int mv_old[2 * IB], mv_new[2 * IB];
+  __shared__ int mv_cnt;
+  __shared__ T stage[2 * IB][LEFT];
+  __shared__ int stage_perm[2 * IB], stage_pid[2 * IB];
+  __shared__ int sing_s;
+  extern __shared__ int pid[];  // m entries: leaf-start row of the row now at each position
+
+  for (int i = t; i < m; i += NTH) pid[i] = i;
+  if (t == 0) sing_s = a.info[mat];
+  __syncthreads();
+
+  for (int jb = 0; jb < w; jb += IB) {
+    const int ib = FULL ? IB : min(IB, w - jb);  // FULL: w is a multiple of IB
+    const int cb = k0 + jb;  // first global column of the block
+    T v[R][IB];
+    int pos[R];
+    // ---- gather the block's columns (lazy: rows still at leaf-start places)
+#pragma unroll
+    for (int s = 0; s < R; ++s) {
+      const int p0 = jb + t + NTH * s;
+      pos[s] = p0 < m ? p0 : (1 << 28);
+      const T* src = A + (int64_t)(k0 + (p0 < m ? pid[p0] : 0)) * n + cb;
+#pragma unroll
+      // the row index is clamped, so the loads need no row guard; padding
+      // slots (pos >= m) are excluded from the pivot search and every write
+      for (int c = 0; c < IB; ++c) v[s][c] = (FULL || c < ib) ? src[c] : T(0);
+    }
+    {
+      // U rows of the block (loads of every pass issued before any use)
+      constexpr int UIT = ((LEAF - IB) * IB + NTH - 1) / NTH > 0 ? ((LEAF - IB) * IB + NTH - 1) / NTH : 1;
+      T u[UIT];
+#pragma unroll
+      for (int i = 0; i < UIT; ++i) {
+        const int e = t + i * NTH, q = e / IB, c = e % IB;
+        const bool in = e < jb * IB && c < ib;
+        u[i] = in ? A[(int64_t)(k0 + pid[in ? q : 0]) * n + cb + c] : T(0);
+      }
+#pragma unroll
+      for (int i = 0; i < UIT; ++i) {
+        const int e = t + i * NTH;
+        if (e < jb * IB) Ublk[e / IB][e % IB] = u[i];
+      }
+    }
+    __syncthreads();
+    // ---- U rows [0, jb) of the block: wave w solves columns w, w+8, ...
+    // (lane = row)
+    if (jb > 0) {
+      for (int cc = wave; cc < ib; cc += NWL) {
+        T x = lane < jb ? Ublk[lane][cc] : T(0);
+        for (int p = 0; p < jb - 1; ++p) {
+          const T xp = readlane<T>(x, p);
+          if (lane > p && lane < jb) x = x - L11[lane][p] * xp;
+        }
+        if (lane < jb) {
+          Ublk[lane][cc] = x;
+          A[(int64_t)(k0 + lane) * n + cb + cc] = x;
+        }
+      }
+    }
+    __syncthreads();
+    // ---- left-looking update with the L columns [k0, cb) of each row, in
+    // chunks of QC columns; the next chunk's loads are in flight while the
+    // current one is applied
+    if (jb > 0) {
+      auto loadc = [&](T (&dst)[R][QC], int q0) {
+#pragma unroll
+        for (int s = 0; s < R; ++s) {
+          const T* lr = A + (int64_t)(k0 + (pos[s] < m ? pos[s] : 0)) * n + k0 + q0;
+#pragma unroll
+          // jb is a multiple of IB and of QC, so a chunk is all inside; rows
+          // past m (clamped to row 0) only feed padding slots, which are
+          // never read back
+          for (int q = 0; q < QC; ++q) dst[s][q] = lr[q];
+        }
+      };
+      auto applyc = [&](const T (&src)[R][QC], int q0) {
+#pragma unroll
+        for (int q = 0; q < QC; ++q) {
+          if (q0 + q >= jb) break;
+#pragma unroll
+          for (int c = 0; c < IB; ++c) {
+            const T u = Ublk[q0 + q][c];
+#pragma unroll
+            for (int s = 0; s < R; ++s) v[s][c] = v[s][c] - src[s][q] * u;
+          }
+        }
+      };
+      T la[R][QC], lb[R][QC];
+      loadc(la, 0);
+      for (int q0 = 0; q0 < jb; q0 += 2 * QC) {
+        if (q0 + QC < jb) loadc(lb, q0 + QC);
+        applyc(la, q0);
+        if (q0 + 2 * QC < jb) loadc(la, q0 + 2 * QC);
+        if (q0 + QC < jb) applyc(lb, q0 + QC);
+      }
+    }
+    // ---- factor the block's columns, one barrier per column
+    int opos[R];
+#pragma unroll
+    for (int s = 0; s < R; ++s) opos[s] = pos[s];
+    // (compile-time column index: one copy of the step per column, so the
+    // register rows are addressed statically)
+    auto fstep = [&]<int c>() __attribute__((always_inline)) {
+      if (c >= ib) return;
+      const int col = jb + c;  // current leaf-relative step
+      const int buf = c & 1;
+      T bv = T(-1);
+      int bp = 1 << 28, bs = -1;
+#pragma unroll
+      for (int s = 0; s < R; ++s) {
+        if (pos[s] >= col && pos[s] < m) {
+          const T av = piv_key(v[s][c], pos[s], col);
+          if (av >= T(0) && piv_better(av, pos[s], bv, bp)) { bv = av; bp = pos[s]; bs = s; }
+        }
+      }
+      T wv = bv;
+      int wp = bp;
+      wave_argmax(wv, wp);
+      if (bs >= 0 && wp == bp) {
+        // this lane owns the wave's candidate row: offer it with its values
+        red_v[buf][wave] = wv;
+        red_p[buf][wave] = wp;
+#pragma unroll
+        for (int s = 0; s < R; ++s)
+          if (s == bs) {
+#pragma unroll
+            for (int j = c; j < IB; ++j) red_row[buf][wave][j] = v[s][j];
+          }
+      } else if (lane == 0 && wv < T(0)) {
+        red_v[buf][wave] = T(-1);
+        red_p[buf][wave] = 1 << 28;
+      }
+      __syncthreads();
+      int wbest = 0;
+      bv = red_v[buf][0];
+      bp = red_p[buf][0];
+#pragma unroll
+      for (int q = 1; q < NWL; ++q) {
+        const T ov = red_v[buf][q];
+        const int op = red_p[buf][q];
+        if (piv_better(ov, op, bv, bp)) { bv = ov; bp = op; wbest = q; }
+      }
+      const int p = bp;
+      if (bv == T(0) && t == 0 && sing_s == 0) sing_s = k0 + col + 1;
+      if (t == 0) ipiv[k0 + col] = k0 + p;
+      const T piv = red_row[buf][wbest][c];
+      // FP32 factors (refined afterwards) scale by one reciprocal per column;
+      // FP64 factors divide, as linalg.cpp:65
+      const T rpiv = sizeof(T) == 4 && piv != T(0) ? T(1) / piv : T(0);
+#pragma unroll
+      for (int s = 0; s < R; ++s) {
+        if (pos[s] == p) pos[s] = col;
+        else if (pos[s] == col) pos[s] = p;
+      }
+#pragma unroll
+      for (int s = 0; s < R; ++s) {
+        if (pos[s] > col && pos[s] < m) {
+          const T l = sizeof(T) == 4 ? v[s][c] * rpiv : (piv != T(0) ? v[s][c] / piv : T(0));
+          v[s][c] = l;
+#pragma unroll
+          for (int j = c + 1; j < IB; ++j) v[s][j] = v[s][j] - l * red_row[buf][wbest][j];
+        }
+      }
+    };
+    [&]<int... C>(std::integer_sequence<int, C...>) __attribute__((always_inline)) {
+      (fstep.template operator()<C>(), ...);
+    }(std::make_integer_sequence<int, IB>{});
+    // ---- L11 rows of the block's new pivot rows: block columns from
+    // registers, left columns (from their pre-block places) loaded by all
+    // threads in parallel
+    // (only blocks before the last: rows and columns below LEFT)
+    const bool need_l11 = jb + ib < w;
+    if (t == 0) mv_cnt = 0;
+    if (need_l11) {
+#pragma unroll
+      for (int s = 0; s < R; ++s) {
+        if (pos[s] >= jb && pos[s] < jb + ib) {
+          stage_pid[pos[s] - jb] = opos[s];
+#pragma unroll
+          for (int c = 0; c < IB; ++c)
+            if (c < ib) L11[pos[s]][jb + c] = v[s][c];
+        }
+      }
+    }
+    __syncthreads();
+    if (need_l11 && jb > 0) {
+      constexpr int LIT = (IB * (LEAF - IB) + NTH - 1) / NTH > 0 ? (IB * (LEAF - IB) + NTH - 1) / NTH : 1;
+      T l[LIT];
+#pragma unroll
+      for (int i = 0; i < LIT; ++i) {
+        const int e = t + i * NTH, rr = e / jb, q = e - rr * jb;
+        l[i] = e < ib * jb ? A[(int64_t)(k0 + stage_pid[rr]) * n + k0 + q] : T(0);
+      }
+#pragma unroll
+      for (int i = 0; i < LIT; ++i) {
+        const int e = t + i * NTH, rr = e / jb, q = e - rr * jb;
+        if (e < ib * jb) L11[jb + rr][q] = l[i];
+      }
+    }
+    __syncthreads();
+    // ---- write the block at final places; list rows whose place changed
+#pragma unroll
+    for (int s = 0; s < R; ++s) {
+      if (pos[s] < m) {
+        T* row = A + (int64_t)(k0 + pos[s]) * n + cb;
+#pragma unroll
+        for (int c = 0; c < IB; ++c)
+          if (c < ib) row[c] = v[s][c];
+        if (pos[s] != opos[s]) {
+          const int e = atomicAdd(&mv_cnt, 1);
+          mv_old[e] = opos[s];
+          mv_new[e] = pos[s];
+        }
+      }
+    }
+    __syncthreads();
+    // ---- move the up-to-date left columns, perm and pid of those rows
+    const int cnt = mv_cnt;
+    const int oc = jb;  // left columns [k0, cb)
+    constexpr int MIT = (2 * IB * (LEAF - IB + 2) + NTH - 1) / NTH;
+    const int tot = cnt * (oc + 2);
+    if (tot > 0) {
+      T mvv[MIT];
+      int mvi[MIT];
+#pragma unroll
+      for (int i = 0; i < MIT; ++i) {
+        const int e = t + i * NTH, q = e / (oc + 2), j = e - q * (oc + 2);
+        const int src = e < tot ? mv_old[q] : 0;
+        mvv[i] = (e < tot && j < oc) ? A[(int64_t)(k0 + src) * n + k0 + j] : T(0);
+        mvi[i] = e < tot ? (j == oc ? perm[k0 + src] : (j > oc ? pid[src] : 0)) : 0;
+      }
+#pragma unroll
+      for (int i = 0; i < MIT; ++i) {
+        const int e = t + i * NTH, q = e / (oc + 2), j = e - q * (oc + 2);
+        if (e < tot) {
+          if (j < oc) stage[q][j] = mvv[i];
+          else if (j == oc) stage_perm[q] = mvi[i];
+          else stage_pid[q] = mvi[i];
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int i = 0; i < MIT; ++i) {
+        const int e = t + i * NTH, q = e / (oc + 2), j = e - q * (oc + 2);
+        if (e < tot) {
+          const int dst = mv_new[q];
+          if (j < oc) A[(int64_t)(k0 + dst) * n + k0 + j] = stage[q][j];
+          else if (j == oc) perm[k0 + dst] = stage_perm[q];
+          else pid[dst] = stage_pid[q];
+        }
+      }
+    }
+    __syncthreads();
+  }
+
+  // Net row moves of this leaf for the columns outside it.
+  if (t == 0) { mv_cnt = 0; a.info[mat] = sing_s; }
+  __syncthreads();
+  int32_t* mv = a.mv + mat * a.mv_stride;
+  for (int i = t; i < m; i += NTH) {
+    const int src = pid[i];
+    if (src != i) {
+      const int e = atomicAdd(&mv_cnt, 1);
+      mv[1 + 2 * e] = k0 + src;
+      mv[2 + 2 * e] = k0 + i;
+    }
+  }
+  __syncthreads();
+  if (t == 0) mv[0] = mv_cnt;
+}
+
+
+// FULL instantiation when every inner block of the panel is IB wide (the
+// column guards then fold away and the block's stores are unconditional).
+template <class T, int R, int IB, int NTH = PT>
+inline void launch_leaf(dim3 grid, size_t shm, hipStream_t s, const LeafArgs<T>& la) {
+  if (la.w % IB == 0) hipLaunchKernelGGL((leaf_kernel<T, R, IB, true, NTH>), grid, dim3(NTH), shm, s, la);
+  else hipLaunchKernelGGL((leaf_kernel<T, R, IB, false, NTH>), grid, dim3(NTH), shm, s, la);
+}
+
+// Leaf launches by panel height m = n - k0 (one workgroup per matrix):
+// thin: stacks of at least four matrices per CU, one or two waves per matrix
+// (m <= 256); wide: 256 or 512 threads (any m up to 8 * 512).
+int leaf_launch_thin(const LeafArgs<float>& la, int m, dim3 grid, size_t shm, hipStream_t s);
+int leaf_launch_thin(const LeafArgs<double>& la, int m, dim3 grid, size_t shm, hipStream_t s);
+int leaf_launch_wide(const LeafArgs<float>& la, int m, dim3 grid, size_t shm, hipStream_t s);
+int leaf_launch_wide(const LeafArgs<double>& la, int m, dim3 grid, size_t shm, hipStream_t s);
+
+}  // namespace npph

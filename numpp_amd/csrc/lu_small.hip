@@ -8,7 +8,8 @@
 // step-k swap of positions k and p (linalg.cpp:58-62) just exchanges two
 // lanes' `pos` values. The pivot search is a wave-wide argmax over
 // (|a[k]|, pos) with NumPP's tie rule (largest magnitude, then smallest current
-// position: linalg.cpp:55-56), the pivot row is broadcast with v_readlane into
+// position: linalg.cpp:55-56; a NaN at position k is kept, a NaN below it is
+// never chosen), the pivot row is broadcast with v_readlane into
 // scalar registers, and each lane applies the rank-1 update to its own row
 // (linalg.cpp:64-68, division by the pivot as in the reference). The solve
 // walks the same lanes: forward substitution with the unit-lower L, then back
@@ -42,7 +43,7 @@ __device__ void lu_wave(const SmallSys& s) {
     double ak = a[0];
 #pragma unroll
     for (int j = 1; j < N; ++j) ak = (j == k) ? a[j] : ak;
-    double v = (act && pos >= k) ? fabs(ak) : -1.0;
+    double v = (act && pos >= k) ? piv_key(ak, pos, k) : -1.0;
     int p = pos;
     wave_argmax(v, p);
     p = __builtin_amdgcn_readfirstlane(p);

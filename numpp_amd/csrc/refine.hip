@@ -685,6 +685,7 @@ extern "C" int npph_gesv_refined_batched_dev(const npph_refine_opts* opts, int64
     npph::set_error("refined solve: B and X must not alias");
     return NPPH_ERR_INVALID;
   }
-  return npph::refined_impl(*opts, count, (int)n, (int)nrhs, A, B, X, berr, iters, info,
-                            stream ? (hipStream_t)stream : npph::default_stream());
+  const hipStream_t s = stream ? (hipStream_t)stream : npph::default_stream();
+  npph::CallScope cs(s);
+  return npph::refined_impl(*opts, count, (int)n, (int)nrhs, A, B, X, berr, iters, info, s);
 }

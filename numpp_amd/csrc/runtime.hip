@@ -5,6 +5,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <unordered_map>
 
 #include "common.hpp"
 
@@ -20,8 +21,6 @@ struct Context {
   int cus = 0;
   char name[256] = "";
   hipStream_t stream = nullptr;
-  std::mutex ws_mu;
-  Workspace ws[kWsCount];
 };
 Context& ctx() {
   static Context c;
@@ -78,36 +77,94 @@ int cu_count() { return ctx().cus; }
 struct WsSet {
   Workspace ws[kWsCount];
   hipStream_t owner = nullptr;
+  bool priv = false;
+  std::mutex mu;  // held by the CallScope that has the set bound
 };
 
 namespace {
 std::atomic<uint64_t> g_ws_gen{0};
 thread_local WsSet* t_ws = nullptr;
+thread_local ScratchArena* t_arena = nullptr;
+thread_local int t_plain = 0;
+
+// Per-stream sets, created on first use and kept until the stream is released.
+std::mutex g_sets_mu;
+std::unordered_map<hipStream_t, WsSet*> g_sets;
+
+WsSet* stream_set(hipStream_t s) {
+  std::lock_guard<std::mutex> g(g_sets_mu);
+  WsSet*& w = g_sets[s];
+  if (!w) {
+    w = new WsSet;
+    w->owner = s;
+  }
+  return w;
 }
+
+void free_set_buffers(WsSet* w) {
+  if (w->owner) (void)hipStreamSynchronize(w->owner);
+  for (auto& x : w->ws)
+    if (x.ptr) {
+      (void)hipFree(x.ptr);
+      x.ptr = nullptr;
+      x.bytes = 0;
+    }
+}
+}  // namespace
 
 WsSet* ws_create(hipStream_t owner) {
   WsSet* w = new WsSet;
   w->owner = owner;
+  w->priv = true;
   return w;
 }
 
 void ws_destroy(WsSet* w) {
   if (!w) return;
-  if (w->owner) (void)hipStreamSynchronize(w->owner);
-  for (auto& x : w->ws)
-    if (x.ptr) (void)hipFree(x.ptr);
+  free_set_buffers(w);
   delete w;
 }
 
-namespace {
-thread_local int t_plain = 0;
+void ws_bind(WsSet* w) { t_ws = w; }
+bool ws_private() { return t_ws != nullptr && t_ws->priv; }
+
+void ws_release_stream(hipStream_t s) {
+  WsSet* w = nullptr;
+  {
+    std::lock_guard<std::mutex> g(g_sets_mu);
+    auto it = g_sets.find(s);
+    if (it == g_sets.end()) return;
+    w = it->second;
+    g_sets.erase(it);
+  }
+  {
+    std::lock_guard<std::mutex> g(w->mu);  // no call is using it any more
+    free_set_buffers(w);
+  }
+  g_ws_gen.fetch_add(1);
+  delete w;
 }
+
+CallScope::CallScope(hipStream_t s) {
+  if (t_ws != nullptr) return;  // a private set, or an enclosing scope
+  set_ = stream_set(s);
+  set_->mu.lock();
+  t_ws = set_;
+}
+
+CallScope::~CallScope() {
+  if (!set_) return;
+  t_ws = nullptr;
+  set_->mu.unlock();
+}
+
+void arena_bind(ScratchArena* a) { t_arena = a; }
+bool arena_bound() { return t_arena != nullptr; }
+
 PlainIssue::PlainIssue() { ++t_plain; }
 PlainIssue::~PlainIssue() { --t_plain; }
-bool plain_issue() { return t_plain > 0; }
+bool plain_issue() { return t_plain > 0 || t_arena != nullptr; }
 
-void ws_bind(WsSet* w) { t_ws = w; }
-WsSet* ws_current() { return t_ws; }
 uint64_t workspace_generation() { return g_ws_gen.load(); }
 
 bool stream_capturing(hipStream_t s) {
@@ -117,17 +174,31 @@ bool stream_capturing(hipStream_t s) {
 }
 
 int workspace(int slot, size_t bytes, void** out) {
-  Context& c = ctx();
   if (slot < 0 || slot >= kWsCount) { set_error("bad workspace slot %d", slot); return NPPH_ERR_INVALID; }
-  std::lock_guard<std::mutex> g(c.ws_mu);
-  Workspace& w = t_ws ? t_ws->ws[slot] : c.ws[slot];
+  if (ScratchArena* a = t_arena) {
+    if (a->len[slot] >= bytes && a->len[slot] > 0) {
+      *out = a->base + a->off[slot];
+      return NPPH_OK;
+    }
+    const size_t want = (bytes + 255) & ~(size_t)255;
+    if (a->used + want > a->bytes) {
+      set_error("scratch: %zu more bytes needed for slot %d, %zu of %zu in use", want, slot, a->used, a->bytes);
+      return NPPH_ERR_INVALID;
+    }
+    a->off[slot] = a->used;
+    a->len[slot] = want;
+    a->used += want;
+    *out = a->base + a->off[slot];
+    return NPPH_OK;
+  }
+  WsSet* set = t_ws;
+  if (!set) { set_error("internal: workspace requested outside a call scope"); return NPPH_ERR_INVALID; }
+  Workspace& w = set->ws[slot];
   if (w.bytes < bytes) {
     g_ws_gen.fetch_add(1);
     if (w.ptr) {
-      // The previous buffer may still be in use by queued work: on the owner
-      // stream of a bound set, on any stream for the process-wide set.
-      if (t_ws && t_ws->owner) NPPH_HIP(hipStreamSynchronize(t_ws->owner));
-      else NPPH_HIP(hipDeviceSynchronize());
+      // the previous buffer may still be in use by work queued on the set's stream
+      NPPH_HIP(hipStreamSynchronize(set->owner));
       NPPH_HIP(hipFree(w.ptr));
       w.ptr = nullptr;
       w.bytes = 0;
@@ -203,6 +274,7 @@ int npph_stream_create(void** stream) {
 }
 int npph_stream_destroy(void* stream) {
   NPPH_TRY(ensure_init());
+  ws_release_stream(static_cast<hipStream_t>(stream));
   NPPH_HIP(hipStreamDestroy(static_cast<hipStream_t>(stream)));
   return NPPH_OK;
 }

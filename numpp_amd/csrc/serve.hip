@@ -146,33 +146,55 @@ void free_resident(Resident* r, hipStream_t s) {
   delete r;
 }
 
-int dmalloc(void** p, size_t bytes, hipStream_t s) {
-  NPPH_HIP(hipMallocAsync(p, bytes ? bytes : 256, s));
-  return NPPH_OK;
-}
+// Stream-ordered device buffers of one request, freed (stream-ordered) when
+// the holder goes out of scope, on every return path, unless handed over with
+// release().
+struct AsyncBufs {
+  hipStream_t s;
+  std::vector<void*> held;
+  explicit AsyncBufs(hipStream_t st) : s(st) {}
+  AsyncBufs(const AsyncBufs&) = delete;
+  AsyncBufs& operator=(const AsyncBufs&) = delete;
+  template <class T>
+  int get(T** out, size_t bytes) {
+    void* p = nullptr;
+    NPPH_HIP(hipMallocAsync(&p, bytes ? bytes : 256, s));
+    held.push_back(p);
+    *out = static_cast<T*>(p);
+    return NPPH_OK;
+  }
+  void release(const void* p) {
+    for (auto& q : held)
+      if (q == p) q = nullptr;
+  }
+  void free_all() {
+    for (void* q : held)
+      if (q) (void)hipFreeAsync(q, s);
+    held.clear();
+  }
+  ~AsyncBufs() { free_all(); }
+};
 
 // Solves one request with resident factors.
 int serve_reuse(Server& sv, Request* r, Resident* res, hipStream_t s) {
-  double *dB = nullptr, *dX = nullptr, *dE = nullptr;
-  int32_t *dI = nullptr, *dF = nullptr;
+  AsyncBufs mem(s);
+  double *dB, *dX, *dE;
+  int32_t *dI, *dF;
   const size_t nb = (size_t)r->n * r->nrhs;
-  int st = NPPH_OK;
-  if ((st = dmalloc((void**)&dB, nb * 8, s)) || (st = dmalloc((void**)&dX, nb * 8, s)) ||
-      (st = dmalloc((void**)&dE, (size_t)r->nrhs * 8, s)) || (st = dmalloc((void**)&dI, 4, s)) ||
-      (st = dmalloc((void**)&dF, 4, s)))
-    return st;
+  NPPH_TRY(mem.get(&dB, nb * 8));
+  NPPH_TRY(mem.get(&dX, nb * 8));
+  NPPH_TRY(mem.get(&dE, (size_t)r->nrhs * 8));
+  NPPH_TRY(mem.get(&dI, 4));
+  NPPH_TRY(mem.get(&dF, 4));
   NPPH_HIP(hipMemcpyAsync(dB, r->B, nb * 8, hipMemcpyHostToDevice, s));
-  st = refine_solve(sv.opts, res->A, res->f, r->nrhs, dB, dX, dE, dI, dF, s);
-  if (st == NPPH_OK) {
-    NPPH_HIP(hipMemcpyAsync(r->X, dX, nb * 8, hipMemcpyDeviceToHost, s));
-    if (r->berr) NPPH_HIP(hipMemcpyAsync(r->berr, dE, (size_t)r->nrhs * 8, hipMemcpyDeviceToHost, s));
-    NPPH_HIP(hipMemcpyAsync(&r->iters, dI, 4, hipMemcpyDeviceToHost, s));
-    NPPH_HIP(hipMemcpyAsync(&r->info, dF, 4, hipMemcpyDeviceToHost, s));
-  }
-  (void)hipFreeAsync(dB, s); (void)hipFreeAsync(dX, s); (void)hipFreeAsync(dE, s);
-  (void)hipFreeAsync(dI, s); (void)hipFreeAsync(dF, s);
+  NPPH_TRY(refine_solve(sv.opts, res->A, res->f, r->nrhs, dB, dX, dE, dI, dF, s));
+  NPPH_HIP(hipMemcpyAsync(r->X, dX, nb * 8, hipMemcpyDeviceToHost, s));
+  if (r->berr) NPPH_HIP(hipMemcpyAsync(r->berr, dE, (size_t)r->nrhs * 8, hipMemcpyDeviceToHost, s));
+  NPPH_HIP(hipMemcpyAsync(&r->iters, dI, 4, hipMemcpyDeviceToHost, s));
+  NPPH_HIP(hipMemcpyAsync(&r->info, dF, 4, hipMemcpyDeviceToHost, s));
+  mem.free_all();
   NPPH_HIP(hipStreamSynchronize(s));
-  return st;
+  return NPPH_OK;
 }
 
 // Factorises and solves a group of same-order requests as one stack. A
@@ -183,15 +205,17 @@ int serve_group(Server& sv, std::vector<Request*>& g, hipStream_t s, Resident** 
   int R = 1;
   for (auto* r : g) R = std::max(R, r->nrhs);
   const size_t nn = (size_t)n * n, nbR = (size_t)n * R;
-  double *dA = nullptr, *dB = nullptr, *dX = nullptr, *dE = nullptr;
-  int32_t *dI = nullptr, *dF = nullptr;
-  void* fbuf = nullptr;
-  int st = NPPH_OK;
-  if ((st = dmalloc((void**)&dA, k * nn * 8, s)) || (st = dmalloc((void**)&dB, k * nbR * 8, s)) ||
-      (st = dmalloc((void**)&dX, k * nbR * 8, s)) || (st = dmalloc((void**)&dE, k * R * 8, s)) ||
-      (st = dmalloc((void**)&dI, k * 4, s)) || (st = dmalloc((void**)&dF, k * 4, s)) ||
-      (st = dmalloc(&fbuf, refine_factor_bytes(sv.opts.precision, k, n), s)))
-    return st;
+  AsyncBufs mem(s);
+  double *dA, *dB, *dX, *dE;
+  int32_t *dI, *dF;
+  void* fbuf;
+  NPPH_TRY(mem.get(&dA, k * nn * 8));
+  NPPH_TRY(mem.get(&dB, k * nbR * 8));
+  NPPH_TRY(mem.get(&dX, k * nbR * 8));
+  NPPH_TRY(mem.get(&dE, k * R * 8));
+  NPPH_TRY(mem.get(&dI, k * 4));
+  NPPH_TRY(mem.get(&dF, k * 4));
+  NPPH_TRY(mem.get(&fbuf, refine_factor_bytes(sv.opts.precision, k, n)));
   for (int64_t i = 0; i < k; ++i) {
     Request* r = g[i];
     NPPH_HIP(hipMemcpyAsync(dA + i * nn, r->A, nn * 8, hipMemcpyHostToDevice, s));
@@ -205,31 +229,25 @@ int serve_group(Server& sv, std::vector<Request*>& g, hipStream_t s, Resident** 
   }
   RefineFactors f;
   refine_factor_carve(sv.opts.precision, k, n, fbuf, f);
-  st = refine_factor(sv.opts, dA, f, s);
-  if (st == NPPH_OK) st = refine_solve(sv.opts, dA, f, R, dB, dX, dE, dI, dF, s);
+  NPPH_TRY(refine_factor(sv.opts, dA, f, s));
+  NPPH_TRY(refine_solve(sv.opts, dA, f, R, dB, dX, dE, dI, dF, s));
   std::vector<int32_t> iters(k), info(k);
   std::vector<double> berr((size_t)k * R);
-  if (st == NPPH_OK) {
-    for (int64_t i = 0; i < k; ++i) {
-      Request* r = g[i];
-      NPPH_HIP(hipMemcpy2DAsync(r->X, (size_t)r->nrhs * 8, dX + i * nbR, (size_t)R * 8, (size_t)r->nrhs * 8, n,
-                                hipMemcpyDeviceToHost, s));
-    }
-    NPPH_HIP(hipMemcpyAsync(berr.data(), dE, berr.size() * 8, hipMemcpyDeviceToHost, s));
-    NPPH_HIP(hipMemcpyAsync(iters.data(), dI, k * 4, hipMemcpyDeviceToHost, s));
-    NPPH_HIP(hipMemcpyAsync(info.data(), dF, k * 4, hipMemcpyDeviceToHost, s));
+  for (int64_t i = 0; i < k; ++i) {
+    Request* r = g[i];
+    NPPH_HIP(hipMemcpy2DAsync(r->X, (size_t)r->nrhs * 8, dX + i * nbR, (size_t)R * 8, (size_t)r->nrhs * 8, n,
+                              hipMemcpyDeviceToHost, s));
   }
+  NPPH_HIP(hipMemcpyAsync(berr.data(), dE, berr.size() * 8, hipMemcpyDeviceToHost, s));
+  NPPH_HIP(hipMemcpyAsync(iters.data(), dI, k * 4, hipMemcpyDeviceToHost, s));
+  NPPH_HIP(hipMemcpyAsync(info.data(), dF, k * 4, hipMemcpyDeviceToHost, s));
   NPPH_HIP(hipStreamSynchronize(s));
-  if (st == NPPH_OK) {
-    for (int64_t i = 0; i < k; ++i) {
-      g[i]->iters = iters[i];
-      g[i]->info = info[i];
-      if (g[i]->berr) std::copy(berr.begin() + i * R, berr.begin() + i * R + g[i]->nrhs, g[i]->berr);
-    }
+  for (int64_t i = 0; i < k; ++i) {
+    g[i]->iters = iters[i];
+    g[i]->info = info[i];
+    if (g[i]->berr) std::copy(berr.begin() + i * R, berr.begin() + i * R + g[i]->nrhs, g[i]->berr);
   }
-  (void)hipFreeAsync(dB, s); (void)hipFreeAsync(dX, s); (void)hipFreeAsync(dE, s);
-  (void)hipFreeAsync(dI, s); (void)hipFreeAsync(dF, s);
-  if (st == NPPH_OK && k == 1 && g[0]->keep && g[0]->key != 0) {
+  if (k == 1 && g[0]->keep && g[0]->key != 0) {
     Resident* res = new Resident;
     res->key = g[0]->key;
     res->n = n;
@@ -237,13 +255,13 @@ int serve_group(Server& sv, std::vector<Request*>& g, hipStream_t s, Resident** 
     res->fbuf = fbuf;
     res->f = f;
     res->bytes = nn * 8 + refine_factor_bytes(sv.opts.precision, 1, n);
+    mem.release(dA);
+    mem.release(fbuf);
     *keep_out = res;
-  } else {
-    (void)hipFreeAsync(dA, s);
-    (void)hipFreeAsync(fbuf, s);
   }
+  mem.free_all();
   NPPH_HIP(hipStreamSynchronize(s));
-  return st;
+  return NPPH_OK;
 }
 
 void worker_main(Server* sv) {

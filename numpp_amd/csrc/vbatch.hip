@@ -384,6 +384,7 @@ int npph_vbatch_plan_create(int64_t count, const int32_t* n, const int32_t* nrhs
     c += nrhs[i];
   }
   hipStream_t s = default_stream();
+  CallScope cs(s);
   int st = NPPH_OK;
   auto up = [&](auto** dst, const auto* src) {
     if (st != NPPH_OK) return;
@@ -440,6 +441,7 @@ int npph_vbatch_execute(void* plan, const double* A, const double* B, double* X,
   Plan* p = static_cast<Plan*>(plan);
   if (!p || !A || !B || !X) { set_error("vbatch: bad arguments"); return NPPH_ERR_INVALID; }
   hipStream_t s = stream ? (hipStream_t)stream : default_stream();
+  CallScope cs(s);
   const bool graphable = use_graph && p->opts.precision == NPPH_PREC_FP64;
   if (!graphable) return run_plan(p, A, B, X, berr, iters, info, s);
   // The eager warm-up and the capture issue the same launches on s.
