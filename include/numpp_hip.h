@@ -124,6 +124,30 @@ int npph_getrs_batched_dev(int64_t count, int64_t n, int64_t nrhs, const double*
 int npph_gesv_batched_dev(int64_t count, int64_t n, int64_t nrhs, double* A, double* B,
                           int32_t* ipiv, int32_t* info, void* stream);
 
+/* ---------------- determinants and inverses ---------------- */
+/* New GpuVTable slots det_batched / slogdet_batched: replace the per-matrix
+ * loops of batched::det and batched::slogdet (src/linalg/batched.cpp:76,
+ * 102-108) over linalg::det / linalg::slogdet (src/linalg/linalg.cpp:393-407,
+ * 408-439) with one device LU of the whole stack. A: count x n x n host buffer
+ * (not modified), dtype F64 or F32 (computed in F64, outputs rounded to the
+ * input dtype as linalg.cpp:20-26). Outputs (each nullable, count entries, in
+ * the input dtype): det = permsign * prod u_ii in row order, or 0 when an
+ * exact-zero pivot appears; sign and logabsdet as slogdet, (0, -inf) when
+ * singular. info (nullable): 0, or the first zero-pivot step + 1. */
+int npph_det_batched(int dtype, int64_t count, int64_t n, const void* A, void* det, void* sign,
+                     void* logabsdet, int32_t* info);
+/* New GpuVTable slot inv_batched: replaces batched::inv (batched.cpp:75) over
+ * linalg::inv / inv_impl (linalg.cpp:105-115, 387-392): LU of each matrix,
+ * then lu_solve against the identity. A not modified; Ainv count x n x n in the
+ * input dtype. info as above; a singular system's Ainv is unspecified (the
+ * reference raises linalg_error("Singular matrix")). */
+int npph_inv_batched(int dtype, int64_t count, int64_t n, const void* A, void* Ainv, int32_t* info);
+/* Device-resident FP64 forms: A (count x n x n, stride n*n) is overwritten by
+ * its packed LU factors; outputs are device arrays. */
+int npph_det_batched_dev(int64_t count, int64_t n, double* A, double* det, double* sign, double* logabsdet,
+                         int32_t* info, void* stream);
+int npph_inv_batched_dev(int64_t count, int64_t n, double* A, double* Ainv, int32_t* info, void* stream);
+
 /* ---------------- mixed precision + FP64 iterative refinement ---------------- */
 /* Options for the refined solver. No reference counterpart (NumPP always
  * factors in FP64, linalg.cpp:20-26); this is the precision addition. */

@@ -56,6 +56,16 @@ struct GpuVTable {
   bool (*gesv_refined_batched)(int precision, DTypeId dt, int64_t count, int64_t n, int64_t nrhs,
                                const void* A, const void* B, void* X, double* berr, int32_t* iters,
                                int32_t* info);
+  // Determinants of a stack (batched::det / linalg::det, batched.cpp:76,
+  // linalg.cpp:393-407): A count x n x n unchanged, det (count, input dtype).
+  bool (*det_batched)(DTypeId dt, int64_t count, int64_t n, const void* A, void* det, int32_t* info);
+  // batched::slogdet / linalg::slogdet (batched.cpp:102-108, linalg.cpp:408-439).
+  bool (*slogdet_batched)(DTypeId dt, int64_t count, int64_t n, const void* A, void* sign, void* logabsdet,
+                          int32_t* info);
+  // batched::inv / linalg::inv (batched.cpp:75, linalg.cpp:105-115, 387-392):
+  // Ainv count x n x n in the input dtype; info > 0 marks a singular matrix,
+  // for which the caller raises linalg_error("Singular matrix").
+  bool (*inv_batched)(DTypeId dt, int64_t count, int64_t n, const void* A, void* Ainv, int32_t* info);
 };
 
 // Null unless a usable gfx950 device is present (gpu_vtable.hpp:56).

@@ -96,6 +96,10 @@ SIGNATURES = {
     "npph_getrf_batched_dev": (_int, [_i64, _i64, _vp, _i64, _vp, _vp, _vp]),
     "npph_getrs_batched_dev": (_int, [_i64, _i64, _i64, _vp, _i64, _vp, _vp, _i64, _vp]),
     "npph_gesv_batched_dev": (_int, [_i64, _i64, _i64, _vp, _vp, _vp, _vp, _vp]),
+    "npph_det_batched": (_int, [_int, _i64, _i64, _vp, _vp, _vp, _vp, _vp]),
+    "npph_inv_batched": (_int, [_int, _i64, _i64, _vp, _vp, _vp]),
+    "npph_det_batched_dev": (_int, [_i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "npph_inv_batched_dev": (_int, [_i64, _i64, _vp, _vp, _vp, _vp]),
     "npph_gesv_refined_batched_dev": (_int, [ctypes.POINTER(RefineOpts), _i64, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "npph_vbatch_plan_create": (_int, [_i64, _vp, _vp, ctypes.POINTER(RefineOpts), _int, ctypes.POINTER(_vp)]),
     "npph_vbatch_plan_destroy": (_int, [_vp]),

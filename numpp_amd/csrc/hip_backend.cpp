@@ -56,9 +56,22 @@ bool gesv_refined_slot(int precision, DTypeId dt, int64_t count, int64_t n, int6
   return true;
 }
 
+bool det_slot(DTypeId dt, int64_t count, int64_t n, const void* A, void* det, int32_t* info) {
+  return npph_det_batched(static_cast<int>(dt), count, n, A, det, nullptr, nullptr, info) == NPPH_OK;
+}
+
+bool slogdet_slot(DTypeId dt, int64_t count, int64_t n, const void* A, void* sign, void* logabsdet,
+                  int32_t* info) {
+  return npph_det_batched(static_cast<int>(dt), count, n, A, nullptr, sign, logabsdet, info) == NPPH_OK;
+}
+
+bool inv_slot(DTypeId dt, int64_t count, int64_t n, const void* A, void* Ainv, int32_t* info) {
+  return npph_inv_batched(static_cast<int>(dt), count, n, A, Ainv, info) == NPPH_OK;
+}
+
 const GpuVTable g_vtable{
     "hip(gfx950)", nullptr, nullptr, nullptr, &gemm_slot, nullptr, nullptr, nullptr,
-    &gesv_batched_slot, &gesv_refined_slot,
+    &gesv_batched_slot, &gesv_refined_slot, &det_slot, &slogdet_slot, &inv_slot,
 };
 
 }  // namespace

@@ -73,6 +73,94 @@ def solve(a, b, forced: Backend = Backend.Auto, cpu_kernel=None):
     return cpu_kernel(a, b).astype(out_dt, copy=False)
 
 
+def _square_stack(a):
+    """Shape checks of require_square (linalg.cpp:30-33) and batched split
+    (batched.cpp:22-30): (lead, count, n)."""
+    if a.ndim < 2 or a.shape[-1] != a.shape[-2]:
+        raise LinAlgError("Last 2 dimensions of the array must be square")
+    if np.iscomplexobj(a):
+        raise NotImplementedError("complex input is served by NumPP's CPU path")
+    lead = a.shape[:-2]
+    count = int(np.prod(lead, dtype=np.int64)) if lead else 1
+    return lead, count, a.shape[-1]
+
+
+def _gpu_target(forced):
+    target = forced if forced != Backend.Auto else env_target()
+    if is_gpu(target) and not gpu_available(target):
+        raise NotImplementedError(f"backend '{backend_name(target)}' is not available in this build")
+    return target
+
+
+def _cpu(name, cpu_kernel, *args):
+    _set_last(Backend.Cpu)
+    if cpu_kernel is None:
+        raise CpuPathSelected(f"{name} dispatched to NumPP's CPU kernel")
+    return cpu_kernel(*args)
+
+
+def det(a, forced: Backend = Backend.Auto, cpu_kernel=None):
+    """numpp::linalg::det (linalg.cpp:393-407; stacks: batched::det,
+    batched.cpp:76) with the HIP det_batched slot: 0 for an exact-zero pivot,
+    output dtype float32 only for float32 input."""
+    a = np.asarray(a)
+    lead, count, n = _square_stack(a)
+    out_dt = _out_dtype(a, a)
+    target = _gpu_target(forced)
+    if is_gpu(target):
+        vt = vtable.get()
+        if vt is not None and vt.det_batched:
+            A = np.ascontiguousarray(a, dtype=np.float64)
+            d = np.empty(count)
+            if vt.det_batched(_lib.DT_F64, count, n, A.ctypes.data, d.ctypes.data, None):
+                _set_last(target)
+                return d.reshape(lead).astype(out_dt, copy=False) if lead else out_dt(d[0])
+    return _cpu("det", cpu_kernel, a)
+
+
+def slogdet(a, forced: Backend = Backend.Auto, cpu_kernel=None):
+    """numpp::linalg::slogdet (linalg.cpp:408-439; stacks: batched.cpp:102-108):
+    (sign, logabsdet), (0, -inf) for an exact-zero pivot."""
+    a = np.asarray(a)
+    lead, count, n = _square_stack(a)
+    out_dt = _out_dtype(a, a)
+    target = _gpu_target(forced)
+    if is_gpu(target):
+        vt = vtable.get()
+        if vt is not None and vt.slogdet_batched:
+            A = np.ascontiguousarray(a, dtype=np.float64)
+            sg = np.empty(count)
+            la = np.empty(count)
+            if vt.slogdet_batched(_lib.DT_F64, count, n, A.ctypes.data, sg.ctypes.data, la.ctypes.data, None):
+                _set_last(target)
+                if lead:
+                    return sg.reshape(lead).astype(out_dt, copy=False), la.reshape(lead).astype(out_dt, copy=False)
+                return out_dt(sg[0]), out_dt(la[0])
+    return _cpu("slogdet", cpu_kernel, a)
+
+
+def inv(a, forced: Backend = Backend.Auto, cpu_kernel=None):
+    """numpp::linalg::inv (linalg.cpp:105-115, 386-392; stacks: batched::inv,
+    batched.cpp:75) with the HIP inv_batched slot; LinAlgError("Singular
+    matrix") for an exact-zero pivot."""
+    a = np.asarray(a)
+    lead, count, n = _square_stack(a)
+    out_dt = _out_dtype(a, a)
+    target = _gpu_target(forced)
+    if is_gpu(target):
+        vt = vtable.get()
+        if vt is not None and vt.inv_batched:
+            A = np.ascontiguousarray(a, dtype=np.float64)
+            out = np.empty((count, n, n))
+            info = np.zeros(count, np.int32)
+            if vt.inv_batched(_lib.DT_F64, count, n, A.ctypes.data, out.ctypes.data, info.ctypes.data):
+                _set_last(target)
+                if np.any(info != 0):
+                    raise LinAlgError("Singular matrix")
+                return out.reshape(a.shape).astype(out_dt, copy=False)
+    return _cpu("inv", cpu_kernel, a)
+
+
 # ---------------------------------------------------------------------------
 # Device-resident API
 
@@ -109,6 +197,22 @@ PRECISIONS = {"fp64": _lib.PREC_FP64, "fp32": _lib.PREC_FP32, "bf16": _lib.PREC_
 def refine_opts(precision="fp32", tol=1e-12, max_iter=10, fallback=True, bf16_terms=6):
     return _lib.RefineOpts(PRECISIONS[precision] if isinstance(precision, str) else int(precision),
                            int(max_iter), float(tol), 1 if fallback else 0, int(bf16_terms))
+
+
+def det_batched_dev(A: DeviceArray, det: DeviceArray | None = None, sign: DeviceArray | None = None,
+                    logabsdet: DeviceArray | None = None, info: DeviceArray | None = None, stream=None):
+    """In place: A (count, n, n) -> LU; per-matrix det / sign / logabsdet (FP64)."""
+    lib = _lib.require_device()
+    count, n, _ = A.shape
+    _lib.check(lib.npph_det_batched_dev(count, n, A.ptr, _ptr(det), _ptr(sign), _ptr(logabsdet), _ptr(info),
+                                        stream))
+
+
+def inv_batched_dev(A: DeviceArray, Ainv: DeviceArray, info: DeviceArray | None = None, stream=None):
+    """A (count, n, n) -> LU in place; Ainv receives the inverses (FP64)."""
+    lib = _lib.require_device()
+    count, n, _ = A.shape
+    _lib.check(lib.npph_inv_batched_dev(count, n, A.ptr, Ainv.ptr, _ptr(info), stream))
 
 
 def gesv_refined_dev(A: DeviceArray, B: DeviceArray, X: DeviceArray, berr: DeviceArray | None = None,

@@ -19,6 +19,9 @@ _dp = ctypes.POINTER(ctypes.c_double)
 GEMM_FN = ctypes.CFUNCTYPE(ctypes.c_bool, _u8, _i64, _i64, _i64, _vp, _vp, _vp)
 GESV_FN = ctypes.CFUNCTYPE(ctypes.c_bool, _u8, _i64, _i64, _i64, _vp, _vp, _vp, _vp)
 GESV_REFINED_FN = ctypes.CFUNCTYPE(ctypes.c_bool, ctypes.c_int, _u8, _i64, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp)
+DET_FN = ctypes.CFUNCTYPE(ctypes.c_bool, _u8, _i64, _i64, _vp, _vp, _vp)
+SLOGDET_FN = ctypes.CFUNCTYPE(ctypes.c_bool, _u8, _i64, _i64, _vp, _vp, _vp, _vp)
+INV_FN = ctypes.CFUNCTYPE(ctypes.c_bool, _u8, _i64, _i64, _vp, _vp, _vp)
 
 
 class GpuVTable(ctypes.Structure):
@@ -33,6 +36,9 @@ class GpuVTable(ctypes.Structure):
         ("separable_corr1d", _vp),
         ("gesv_batched", GESV_FN),
         ("gesv_refined_batched", GESV_REFINED_FN),
+        ("det_batched", DET_FN),
+        ("slogdet_batched", SLOGDET_FN),
+        ("inv_batched", INV_FN),
     ]
 
 

@@ -55,6 +55,12 @@ def _load():
     lib.orc_dot2_entry.restype = ctypes.c_double
     lib.orc_absdot_entry.argtypes = [ctypes.c_int64, ctypes.c_int64, _dp, _dp, ctypes.c_int64, ctypes.c_int64]
     lib.orc_absdot_entry.restype = ctypes.c_double
+    lib.orc_det.argtypes = [_dp, ctypes.c_int]
+    lib.orc_det.restype = ctypes.c_double
+    lib.orc_slogdet.argtypes = [_dp, ctypes.c_int, _dp, _dp]
+    lib.orc_slogdet.restype = None
+    lib.orc_inv.argtypes = [_dp, ctypes.c_int, _dp]
+    lib.orc_inv.restype = ctypes.c_int
     lib.orc_backward_error.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp]
     lib.orc_backward_error.restype = None
     _lib = lib
@@ -124,6 +130,31 @@ def batched_solve(a, b):
     info = np.zeros(count, np.int32)
     _load().orc_batched_solve(count, n, m, _ptr(a), _ptr(b), _ptr(x), _ptr(info, _ip))
     return x, info
+
+
+def det(a):
+    """linalg.cpp:393-407 for one (n, n) matrix (0.0 when singular)."""
+    a = _c(a)
+    return _load().orc_det(_ptr(a), a.shape[0])
+
+
+def slogdet(a):
+    """linalg.cpp:408-439 (real): (sign, logabsdet); (0.0, -inf) when singular."""
+    a = _c(a)
+    s, la = ctypes.c_double(), ctypes.c_double()
+    _load().orc_slogdet(_ptr(a), a.shape[0], ctypes.byref(s), ctypes.byref(la))
+    return s.value, la.value
+
+
+def inv(a):
+    """linalg.cpp:105-115 for one (n, n) matrix; raises SingularMatrix."""
+    a = _c(a)
+    n = a.shape[0]
+    out = np.empty((n, n))
+    s = _load().orc_inv(_ptr(a), n, _ptr(out))
+    if s >= 0:
+        raise SingularMatrix(f"Singular matrix (exact zero pivot at step {s})")
+    return out
 
 
 def matmul(a, b):

@@ -14,6 +14,9 @@
  *   orc_batched_solve  <- src/linalg/batched.cpp:90-100 (batched::solve loop)
  *   orc_matmul         <- src/backend/backend.cpp:55-84 (cpu_matmul, f64)
  *   orc_matmul_f32     <- src/backend/backend.cpp:55-84 (cpu_matmul, f32)
+ *   orc_det            <- src/linalg/linalg.cpp:393-407 (det)
+ *   orc_slogdet        <- src/linalg/linalg.cpp:408-439 (slogdet, real case)
+ *   orc_inv            <- src/linalg/linalg.cpp:105-115 (inv_impl)
  * Helpers that NumPP does not have (pinned by mathematics, not by NumPP):
  *   orc_dot2_entry     compensated dot product (Ogita-Rump-Oishi Dot2) of one
  *                      GEMM entry, used to bound GEMM error at sizes where the
@@ -33,6 +36,11 @@
  *     46-55), the reference's frozen golden tests/golden/g15653376641833044941
  *     .npy, numpy goldens, exact integer products.
  *   dot2_entry: exact rational products on integer-valued inputs.
+ *   det / slogdet / inv: numpy.linalg goldens (tests/golden/det_cases.npz);
+ *     exact rational determinants and inverses for n <= 4; identity,
+ *     permutation (sign of the permutation exactly) and triangular matrices
+ *     (product of the diagonal exactly); exact-zero singular inputs (det 0,
+ *     slogdet (0, -inf), inv singular); A inv(A) = I.
  */
 #include <math.h>
 #include <stdint.h>
@@ -107,6 +115,67 @@ int orc_solve(const double* A, const double* B, double* X, double* LU, int* step
   int sing = orc_lu_decompose(W, n, piv, steps, &sgn);
   if (sing < 0) orc_lu_solve(W, n, piv, X, m);
   if (LU) memcpy(LU, W, sizeof(double) * (size_t)n * (size_t)n);
+  free(W); free(piv);
+  return sing;
+}
+
+/* linalg.cpp:393-407: the determinant from a copy of A: 0 when lu_decompose
+ * reports singular, else permsign times the diagonal of U, multiplied in row
+ * order starting from the sign. */
+double orc_det(const double* A, int n) {
+  double* W = (double*)malloc(sizeof(double) * (size_t)(n > 0 ? n * n : 1));
+  int* piv = (int*)malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
+  memcpy(W, A, sizeof(double) * (size_t)n * (size_t)n);
+  double sgn = 1.0, d;
+  if (orc_lu_decompose(W, n, piv, NULL, &sgn) >= 0) {
+    d = 0.0;
+  } else {
+    d = sgn;
+    for (int i = 0; i < n; ++i) d *= W[(int64_t)i * n + i];
+  }
+  free(W); free(piv);
+  return d;
+}
+
+/* linalg.cpp:408-439 for real matrices: singular -> sign 0, logabsdet -inf;
+ * else logabsdet = sum of log|u_ii| and sign = permsign times the product of
+ * u_ii / |u_ii|, both in row order. */
+void orc_slogdet(const double* A, int n, double* sign, double* logabsdet) {
+  double* W = (double*)malloc(sizeof(double) * (size_t)(n > 0 ? n * n : 1));
+  int* piv = (int*)malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
+  memcpy(W, A, sizeof(double) * (size_t)n * (size_t)n);
+  double sgn = 1.0;
+  if (orc_lu_decompose(W, n, piv, NULL, &sgn) >= 0) {
+    *sign = 0.0;
+    *logabsdet = -INFINITY;
+  } else {
+    double la = 0.0, sg = sgn;
+    for (int i = 0; i < n; ++i) {
+      const double d = W[(int64_t)i * n + i];
+      const double m = fabs(d);
+      la += log(m);
+      sg *= d / m;
+    }
+    *sign = sg;
+    *logabsdet = la;
+  }
+  free(W); free(piv);
+}
+
+/* linalg.cpp:105-115: factor a copy of A and solve against the identity (n
+ * right-hand-side columns). Returns -1 on success or the singular step (the
+ * reference raises linalg_error("Singular matrix")). */
+int orc_inv(const double* A, int n, double* Ainv) {
+  double* W = (double*)malloc(sizeof(double) * (size_t)(n > 0 ? n * n : 1));
+  int* piv = (int*)malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
+  memcpy(W, A, sizeof(double) * (size_t)n * (size_t)n);
+  double sgn;
+  const int sing = orc_lu_decompose(W, n, piv, NULL, &sgn);
+  if (sing < 0) {
+    for (int64_t i = 0; i < (int64_t)n * n; ++i) Ainv[i] = 0.0;
+    for (int i = 0; i < n; ++i) Ainv[(int64_t)i * n + i] = 1.0;
+    orc_lu_solve(W, n, piv, Ainv, n);
+  }
   free(W); free(piv);
   return sing;
 }

@@ -12,6 +12,9 @@ Fixtures (all .npz / .npy, loadable with allow_pickle=False):
   config1_x.npy       np.linalg.solve on the 256 x 64 x 64 stacked workload
                       (A = U[-1,1) + 64 I, b ~ N(0,1), default_rng(0)).
   matmul_cases.npz    seeded float64 / float32 GEMMs with numpy results.
+  det_cases.npz       seeded general matrices (rows shuffled so pivoting
+                      swaps) with np.linalg.det, np.linalg.slogdet and
+                      np.linalg.inv (LAPACK dgetrf / dgesv).
   numpp_g15653376641833044941.npy
                       the reference's own frozen matmul golden
                       (arange(1,21).reshape(4,5) @ arange(2,17).reshape(5,3)),
@@ -81,6 +84,23 @@ def matmul_cases():
     np.savez(os.path.join(HERE, "matmul_cases.npz"), **out)
 
 
+def det_cases():
+    rng = np.random.default_rng(2025)
+    sizes = [1, 2, 3, 5, 8, 16, 33, 64, 65, 100, 129]
+    out = {}
+    for n in sizes:
+        a = rng.uniform(-1.0, 1.0, (n, n)) + np.diag(rng.uniform(1.0, 2.0, n))
+        a = a[rng.permutation(n)]
+        out[f"A_{n}"] = a
+        out[f"det_{n}"] = np.array(np.linalg.det(a))
+        sign, logabs = np.linalg.slogdet(a)
+        out[f"sign_{n}"] = np.array(sign)
+        out[f"logabsdet_{n}"] = np.array(logabs)
+        out[f"inv_{n}"] = np.linalg.inv(a)
+    out["sizes"] = np.array(sizes, np.int32)
+    np.savez(os.path.join(HERE, "det_cases.npz"), **out)
+
+
 def reference_golden():
     src = "/root/reference/tests/golden/g15653376641833044941.npy"
     dst = os.path.join(HERE, "numpp_g15653376641833044941.npy")
@@ -93,5 +113,6 @@ if __name__ == "__main__":
     gesv_cases()
     config1()
     matmul_cases()
+    det_cases()
     reference_golden()
     print("goldens written to", HERE)

@@ -225,3 +225,85 @@ def test_planted_mistake_is_caught():
     col = np.abs(a[:, 0])
     last_max = int(np.flatnonzero(col == col.max())[-1])
     assert last_max != steps[0]
+
+
+# ---------------- det / slogdet / inv: linalg.cpp:105-115, 393-439 ----------------
+
+def _exact_det_inv(a):
+    """Determinant and inverse in exact rationals (Gauss-Jordan)."""
+    n = len(a)
+    A = [[Fraction(v) for v in row] + [Fraction(int(i == j)) for j in range(n)] for i, row in enumerate(a)]
+    det = Fraction(1)
+    for k in range(n):
+        p = next((i for i in range(k, n) if A[i][k] != 0), None)
+        if p is None:
+            return Fraction(0), None
+        if p != k:
+            A[k], A[p] = A[p], A[k]
+            det = -det
+        det *= A[k][k]
+        piv = A[k][k]
+        A[k] = [v / piv for v in A[k]]
+        for i in range(n):
+            if i != k and A[i][k] != 0:
+                f = A[i][k]
+                A[i] = [vi - f * vk for vi, vk in zip(A[i], A[k])]
+    return det, [row[n:] for row in A]
+
+
+def test_det_inv_numpy_goldens(golden):
+    g = golden("det_cases.npz")
+    for n in g["sizes"]:
+        a = g[f"A_{n}"]
+        d = oracle.det(a)
+        assert abs(d - g[f"det_{n}"]) <= 1e-12 * n * abs(g[f"det_{n}"])
+        s, la = oracle.slogdet(a)
+        assert s == g[f"sign_{n}"] and abs(la - g[f"logabsdet_{n}"]) <= 1e-12 * n * max(1.0, abs(la))
+        assert abs(d - s * np.exp(la)) <= 1e-12 * n * abs(d)
+        ai = oracle.inv(a)
+        np.testing.assert_allclose(ai, g[f"inv_{n}"], rtol=0, atol=1e-11 * np.abs(g[f"inv_{n}"]).max())
+        np.testing.assert_allclose(a @ ai, np.eye(n), rtol=0, atol=1e-12 * n)
+
+
+def test_det_inv_exact_small():
+    rng = np.random.default_rng(11)
+    for n in (1, 2, 3, 4):
+        for _ in range(25):
+            a = rng.integers(-9, 10, (n, n)).astype(np.float64)
+            de, ie = _exact_det_inv(a.tolist())
+            d = oracle.det(a)
+            s, la = oracle.slogdet(a)
+            if de == 0:
+                assert d == 0.0 and s == 0.0 and la == -np.inf
+                with pytest.raises(oracle.SingularMatrix):
+                    oracle.inv(a)
+                continue
+            assert abs(d - float(de)) <= 1e-13 * abs(float(de))
+            assert s == (1.0 if de > 0 else -1.0)
+            assert abs(la - np.log(abs(float(de)))) <= 1e-13 * max(1.0, abs(la))
+            xe = np.array([[float(v) for v in row] for row in ie])
+            assert np.all(np.abs(oracle.inv(a) - xe) <= 1e-13 * np.maximum(1.0, np.abs(xe)))
+
+
+def test_det_special_cases():
+    for n in (0, 1, 6, 40):
+        assert oracle.det(np.eye(n)) == 1.0 and oracle.slogdet(np.eye(n)) == (1.0, 0.0)
+    # permutation matrices: the sign of the permutation, exactly
+    rng = np.random.default_rng(12)
+    for n in (2, 5, 9):
+        p = rng.permutation(n)
+        inversions = sum(1 for i in range(n) for j in range(i + 1, n) if p[i] > p[j])
+        assert oracle.det(np.eye(n)[p]) == (-1.0) ** inversions
+    # upper triangular: no row swaps, the diagonal product in row order
+    t = np.triu(rng.uniform(-1, 1, (7, 7))) + np.diag(rng.uniform(1, 2, 7))
+    want = 1.0
+    for v in np.diag(t):
+        want *= v
+    assert oracle.det(t) == want
+    s, la = oracle.slogdet(t)
+    assert s == np.sign(want) and la == sum(np.log(np.abs(np.diag(t))).tolist())
+    # exact-zero singularity (linalg.cpp:57) -> 0, (0, -inf), inv raises
+    a = np.array([[1.0, 2.0, 3.0], [2.0, 4.0, 6.0], [1.0, 0.0, 1.0]])
+    assert oracle.det(a) == 0.0 and oracle.slogdet(a) == (0.0, -np.inf)
+    with pytest.raises(oracle.SingularMatrix):
+        oracle.inv(a)
