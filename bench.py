@@ -1,7 +1,7 @@
 """Benchmark of the HIP stacked solve on one MI355X.
 
-Default workload (the headline): mixed-precision refined solve of a stack of
-64 systems of order 2048 with 8 right-hand sides each, cond_2(A) = 1e4:
+Headline (the one JSON line): mixed-precision refined solve of a stack of 64
+systems of order 2048 with 8 right-hand sides each, cond_2(A) = 1e4:
 A = U diag(s) V^T with U, V Haar orthogonal (QR of standard-normal matrices,
 sign-fixed), s = logspace(0, -4, 2048); B ~ N(0, 1); PCG64 seed 3. The 64
 matrices pair 8 Haar U with 8 Haar V (A_{8i+j} = U_i S V_j^T), which keeps the
@@ -13,13 +13,17 @@ right-hand side, with the backward error reported per column. The same step
 in plain FP64 (LU + getrs + backward-error report) is timed beside it.
 `value` = systems solved per second by the faster mixed-precision mode.
 
-Other workloads (--workload): gemm (8192^3 FP64 DGEMM, TFLOP/s), plain64
-(256 x 64 x 64 FP64 stacked solve, solves/s), uneven (20,000 systems of
-n in [8, 256], FP64 and FP32 + refinement), serve (Poisson-arrival serving
-loop, p50/p99 latency and solves/s per rate), copy (device copy, the
-calibration workload for the HBM byte counters).
+The default run (--workload all) also times the other four workloads and nests
+their records under "configs": plain64 (256 x 64 x 64 FP64 stacked solve,
+solves/s), gemm (8192^3 FP64 DGEMM, TFLOP/s), uneven (20,000 systems of n in
+[8, 256], FP64 and FP32 + refinement), serve (Poisson-arrival serving loop,
+p50/p99 latency and solves/s per rate). Each record carries its own roofline
+(dominant kernel class, per-launch HIP events) and CPU baseline (NumPP's
+portable kernel restated, kernel-only and with NumPP's ndarray copies, one
+pinned core). --workload X runs one of them (or copy: the calibration workload
+for the HBM byte counters) as the single line.
 
-Prints ONE JSON line. --gpus above 1 exits: this build runs on one GPU.
+--gpus above 1 exits: this build runs on one GPU.
 """
 from __future__ import annotations
 
@@ -71,6 +75,42 @@ def pin_one_core():
 def unpin(cpus):
     if cpus:
         os.sched_setaffinity(0, set(cpus))
+
+
+def cpu_timed(units, kernel_fn, numpp_fn=None):
+    """Times the oracle kernel on each unit i (kernel_fn(i)) and, interleaved
+    unit by unit so that both see the same cache and clock state, its
+    NumPP-equivalent form with NumPP's ndarray copies (numpp_fn(i)), on one
+    pinned host core. Returns (seconds, numpp_seconds)."""
+    cpus = pin_one_core()
+    dt, dt2 = 0.0, (0.0 if numpp_fn is not None else None)
+    try:
+        for i in units:
+            t1 = time.perf_counter()
+            kernel_fn(i)
+            dt += time.perf_counter() - t1
+            if numpp_fn is not None:
+                t2 = time.perf_counter()
+                numpp_fn(i)
+                dt2 += time.perf_counter() - t2
+    finally:
+        unpin(cpus)
+    return dt, dt2
+
+
+def cpu_record(units, unit, dt, dt2, sample, numpp_units=None, extra=None):
+    model, ncpu = cpu_info()
+    r = {"value": units / dt, "unit": unit, "cores": 1, "kind": "port",
+         "sample": f"{sample}; oracle C restatement of NumPP's portable kernel, -O3 -ffp-contract=off, "
+                   f"1 of {ncpu} host cores ({model})",
+         "seconds": dt}
+    if dt2 is not None:
+        r["numpp_equivalent_value"] = (numpp_units if numpp_units is not None else units) / dt2
+        r["numpp_equivalent_note"] = ("the same kernels plus the ndarray copies NumPP makes around them "
+                                      "(ndarray copy/astype element loops, to_vec/from_vec, stack)")
+    if extra:
+        r.update(extra)
+    return r
 
 
 def haar(rng, n):
@@ -246,21 +286,19 @@ def bench_mixed(args):
     # restated) on one core for a bounded sample of the same workload.
     cpu = None
     if not args.no_cpu:
-        A0 = dA.to_host()[: args.cpu_sample]
-        cpus = pin_one_core()
-        t1 = time.perf_counter()
-        xo = [oracle.solve(A0[i], B[i]) for i in range(args.cpu_sample)]
-        dt = time.perf_counter() - t1
-        unpin(cpus)
-        model, ncpu = cpu_info()
-        cpu = {"value": args.cpu_sample / dt, "unit": "solves/s", "cores": 1, "kind": "port",
-               "sample": f"{args.cpu_sample} of {count} systems (n={n}, 8 RHS), FP64 LU+solve, "
-                         f"oracle C restatement -O3 -ffp-contract=off, 1 of {ncpu} cores ({model})",
-               "seconds": dt}
+        k = args.cpu_sample
+        A0 = dA.to_host()[:k]
+        xo = [None] * k
+
+        def kern(i):
+            xo[i] = oracle.solve(A0[i], B[i])
+        dt, dt2 = cpu_timed(range(k), kern, lambda i: oracle.batched_solve_numpp(A0[i:i + 1], B[i:i + 1]))
+        cpu = cpu_record(k, "solves/s", dt, dt2, f"{k} of {count} systems (n={n}, 8 RHS), FP64 LU+solve, "
+                         f"each also timed with NumPP's copies")
         xg = x64 if x64 is not None else results[best]["x"]
-        fe = max(float(np.abs(xg[i] - xo[i]).max() / np.abs(xo[i]).max()) for i in range(args.cpu_sample))
+        fe = max(float(np.abs(xg[i] - xo[i]).max() / np.abs(xo[i]).max()) for i in range(k))
         results["fp64" if x64 is not None else best]["fwd_err_vs_oracle_sample"] = fe
-        log(f"cpu baseline: {dt:.2f}s for {args.cpu_sample} systems; gpu vs oracle fwd err {fe:.2e}")
+        log(f"cpu baseline: {dt:.2f}s for {k} systems; gpu vs oracle fwd err {fe:.2e}")
 
     r = results[best]
     line = {
@@ -326,15 +364,8 @@ def bench_gemm(args):
         m = 2048
         r2 = np.random.default_rng(1)
         a2, b2 = r2.standard_normal((m, m)), r2.standard_normal((m, m))
-        cpus = pin_one_core()
-        t1 = time.perf_counter()
-        oracle.matmul(a2, b2)
-        dt = time.perf_counter() - t1
-        unpin(cpus)
-        model, ncpu = cpu_info()
-        cpu = {"value": 2.0 * m ** 3 / dt / 1e12, "unit": "TFLOP/s", "cores": 1, "kind": "port",
-               "sample": f"2048^3 FP64 GEMM, oracle i-p-j loop -O3 -ffp-contract=off, 1 of {ncpu} cores ({model})",
-               "seconds": dt}
+        dt, dt2 = cpu_timed(range(1), lambda i: oracle.matmul(a2, b2), lambda i: oracle.matmul_numpp(a2, b2))
+        cpu = cpu_record(2.0 * m ** 3 / 1e12, "TFLOP/s", dt, dt2, "2048^3 FP64 GEMM (i-p-j loop), seed 1")
     return {"metric": "FP64 DGEMM TFLOP/s (8192^3, device-resident)", "value": flops / (ms / 1e3) / 1e12,
             "unit": "TFLOP/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
@@ -379,22 +410,15 @@ def bench_plain64(args):
     roof = roofline_from(prof, "lu_wave64")
     x = dB.to_host()
     xo, _ = oracle.batched_solve(a, b)
-    cpus = pin_one_core()
-    t1 = time.perf_counter()
     reps = 20
-    for _ in range(reps):
-        oracle.batched_solve(a, b)
-    dt = (time.perf_counter() - t1) / reps
-    unpin(cpus)
-    model, ncpu = cpu_info()
+    dt, dt2 = cpu_timed(range(reps), lambda i: oracle.batched_solve(a, b), lambda i: oracle.batched_solve_numpp(a, b))
+    cpu = cpu_record(256 * reps, "solves/s", dt, dt2, f"the full 256-system batch, {reps} repetitions")
     return {"metric": "FP64 stacked solve solves/s (256 x 64x64, 1 RHS)", "value": 256 / (ms / 1e3),
             "unit": "solves/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
             "config": {"workload": "256 x 64x64 float64, U[-1,1)+64I, 1 RHS, seed 0", "parallelism": "single-gpu"},
             "rel_err_vs_oracle": float(np.abs(x - xo).max() / np.abs(xo).max()),
-            "roofline": roof,
-            "cpu_baseline": {"value": 256 / dt, "unit": "solves/s", "cores": 1, "kind": "port",
-                             "sample": f"full 256-system batch, oracle, 1 of {ncpu} cores ({model})"}}
+            "roofline": roof, "cpu_baseline": cpu}
 
 def bench_uneven(args):
     """Uneven batch: --systems independent float64 systems (default 20,000),
@@ -479,19 +503,14 @@ def bench_uneven(args):
         p.close()
     cpu = None
     if not args.no_cpu:
-        idx = np.arange(S)
-        cpus = pin_one_core()
-        t1 = time.perf_counter()
-        for i in idx:
+        def one(i, f):
             n, r = int(ns[i]), int(nr[i])
-            oracle.solve(A[offA[i]:offA[i + 1]].reshape(n, n), B[offB[i]:offB[i + 1]].reshape(n, r))
-        dt = time.perf_counter() - t1
-        unpin(cpus)
-        model, ncpu = cpu_info()
-        cpu = {"value": len(idx) / dt, "unit": "solves/s", "cores": 1, "kind": "port",
-               "sample": f"all {S} systems, FP64 LU+solve, oracle C restatement -O3 -ffp-contract=off, "
-                         f"1 of {ncpu} cores ({model})", "seconds": dt}
-        log(f"cpu baseline: {dt:.2f}s for {len(idx)} systems")
+            return f(A[offA[i]:offA[i + 1]].reshape(1, n, n), B[offB[i]:offB[i + 1]].reshape(1, n, r))
+        dt, dt2 = cpu_timed(range(S), lambda i: one(i, oracle.batched_solve),
+                            lambda i: one(i, oracle.batched_solve_numpp))
+        cpu = cpu_record(S, "solves/s", dt, dt2, f"all {S} systems, FP64 LU+solve, each also timed with NumPP's "
+                         f"copies")
+        log(f"cpu baseline: {dt:.2f}s for {S} systems")
     r = results[best]
     return {"metric": "solves/s + p99 latency, mixed-prec refined vs FP64 LU at backward err <=1e-12",
             "value": r["solves_per_s"], "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
@@ -508,62 +527,90 @@ def bench_uneven(args):
             "cpu_baseline": cpu}
 
 
-def bench_serve(args):
-    """Serving loop under Poisson arrivals at each rate in --rates (requests/s).
-    --requests requests (default 160), PCG64 seed 5: n = exp(U(ln 16, ln 4096)),
-    nrhs = round(exp(U(0, ln 64))), kappa = exp(U(ln 1e2, ln 1e8)); A = U diag(s)
-    V^T with Haar U, V and s = logspace(0, -log10 kappa, n). With probability
-    0.3 a request (after the first) reuses the factors kept by a uniformly
-    chosen earlier request with new right-hand sides; every other request keeps
-    its factors. Budget 32e9 bytes (32 GB = 29.8 GiB), 4 workers, batches of up
-    to 8 same-order requests. Mixed (FP32 + refinement) and plain FP64."""
-    import oracle
-    from numpp_amd.serve import Server
+def serve_trace(R, distinct, seed=5):
+    """The serving trace: R requests cycling over `distinct` generated systems
+    (PCG64 `seed`): n = exp(U(ln 16, ln 4096)), nrhs = round(exp(U(0, ln 64))),
+    kappa = exp(U(ln 1e2, ln 1e8)); A = U diag(s) V^T with Haar U, V and
+    s = logspace(0, -log10 kappa, n) (the product on the device). With
+    probability 0.3 a request (after the first of its cycle) reuses the factors
+    kept by a uniformly chosen earlier request of the same cycle, with new
+    right-hand sides; every other request keeps its factors under a key of its
+    own. Cycling keeps the host-side QR cost of the Haar factors bounded."""
+    import numpp_amd
+    from numpp_amd import _lib
+    from numpp_amd.device import DeviceArray
 
-    numpp_amd_lib = __import__("numpp_amd").require_device()
-    del numpp_amd_lib
-    R = args.requests
-    rng = np.random.default_rng(5)
-    t0 = time.time()
-    ns = np.exp(rng.uniform(np.log(16), np.log(4096), R)).astype(int)
-    nrs = np.maximum(1, np.rint(np.exp(rng.uniform(0.0, np.log(64), R)))).astype(int)
-    kap = np.exp(rng.uniform(np.log(1e2), np.log(1e8), R))
-    reuse_u = rng.uniform(size=R)
-    owners = []
-    mats, rhss, keys, keeps, owner_of = [], [], [], [], []
-    for i in range(R):
+    lib = numpp_amd.require_device()
+    rng = np.random.default_rng(seed)
+    D = min(R, distinct)
+    ns = np.exp(rng.uniform(np.log(16), np.log(4096), D)).astype(int)
+    nrs = np.maximum(1, np.rint(np.exp(rng.uniform(0.0, np.log(64), D)))).astype(int)
+    kap = np.exp(rng.uniform(np.log(1e2), np.log(1e8), D))
+    reuse_u = rng.uniform(size=D)
+    owners, mats, rhss, owner_of = [], [], [], []
+    for i in range(D):
         if i > 0 and reuse_u[i] < 0.3 and owners:
             j = owners[int(rng.integers(0, len(owners)))]
             n = ns[j]
             mats.append(None)
-            keys.append(j + 1)
-            keeps.append(0)
             owner_of.append(j)
         else:
-            n = ns[i]
+            n = int(ns[i])
             s = np.logspace(0.0, -np.log10(kap[i]), n)
-            a = (haar(rng, n) * s) @ haar(rng, n).T
-            mats.append(a)
-            keys.append(i + 1)
-            keeps.append(1)
+            us = DeviceArray.from_host(haar(rng, n) * s)
+            vt = DeviceArray.from_host(np.ascontiguousarray(haar(rng, n).T))
+            da = DeviceArray((n, n))
+            _lib.check(lib.npph_dgemm_dev(n, n, n, 1.0, us.ptr, n, 0, vt.ptr, n, 0, 0.0, da.ptr, n, 0, 1, None))
+            mats.append(da.to_host())
             owner_of.append(i)
             owners.append(i)
         rhss.append(rng.standard_normal((n, nrs[i])))
+    idx = [i % D for i in range(R)]
+    cyc = [i // D for i in range(R)]
+    keys = [owner_of[j] + 1 + c * D for j, c in zip(idx, cyc)]
+    keeps = [0 if mats[j] is None else 1 for j in idx]
+    return (np.array([rhss[j].shape[0] for j in idx]), [mats[j] for j in idx], [rhss[j] for j in idx], keys,
+            keeps, [owner_of[j] + c * D for j, c in zip(idx, cyc)])
+
+
+def bench_serve(args):
+    """Serving loop under Poisson arrivals at each rate in --rates (requests/s),
+    the same --requests requests (cycling over --distinct generated systems)
+    replayed per rate (serve_trace). Budget 32e9
+    bytes (32 GB = 29.8 GiB), 4 workers, batches of up to 8 same-bucket
+    requests. Mixed (FP32 + refinement) and plain FP64. The roofline is the
+    dominant kernel class of one extra replay at the lowest rate with per-launch
+    HIP events."""
+    import ctypes
+
+    import oracle
+    import numpp_amd
+    from numpp_amd.serve import Server
+
+    lib = numpp_amd.require_device()
+    R = args.requests
+    t0 = time.time()
+    ns, mats, rhss, keys, keeps, owner_of = serve_trace(R, args.distinct)
     nre = sum(m is None for m in mats)
-    log(f"inputs: {R} requests ({nre} reuse), n {ns.min()}..{ns.max()}, in {time.time() - t0:.1f}s")
+    log(f"inputs: {R} requests over {min(R, args.distinct)} systems ({nre} reuse), n {ns.min()}..{ns.max()}, "
+        f"in {time.time() - t0:.1f}s")
     rates = [float(x) for x in args.rates.split(",")]
+    precs = [m for m in ("fp32", "fp64") if m in args.modes.split(",")] or ["fp32", "fp64"]
     runs = []
-    for prec in [m for m in ("fp32", "fp64") if m in args.modes.split(",")]:
-        # warm-up replay: compiles nothing, but sizes pools and workspaces
+
+    def arrivals(lam):
+        gaps = np.random.default_rng(50 + int(lam)).exponential(1.0 / lam, R)
+        return np.cumsum(gaps) - gaps[0]
+
+    for prec in precs:
+        # warm-up replay: sizes pools and workspaces
         srv = Server(budget_bytes=32e9, workers=4, max_batch=8, precision=prec)
         k = min(R, 24)
         srv.replay(np.zeros(k), mats[:k], rhss[:k], keys[:k], keeps[:k])
         srv.close()
         for lam in rates:
-            gaps = np.random.default_rng(50 + int(lam)).exponential(1.0 / lam, R)
-            t_arr = np.cumsum(gaps) - gaps[0]
             srv = Server(budget_bytes=32e9, workers=4, max_batch=8, precision=prec)
-            out = srv.replay(t_arr, mats, rhss, keys, keeps)
+            out = srv.replay(arrivals(lam), mats, rhss, keys, keeps)
             st = srv.stats()
             srv.close()
             ok = out["req_status"] == 0
@@ -578,15 +625,27 @@ def bench_serve(args):
             log(f"{prec} rate {lam:g}/s: {run['solves_per_s']:.1f} solves/s, p50 {run['p50_ms']:.1f} ms, "
                 f"p99 {run['p99_ms']:.1f} ms, berr_max {berr:.2e}, ok {run['ok']}/{R}, "
                 f"fallbacks {run['fallbacks']}, batched {run['batched']}")
+    # kernel classes of one replay (headline precision, lowest rate)
+    lib.npph_prof_reset()
+    lib.npph_prof_enable(1)
+    srv = Server(budget_bytes=32e9, workers=4, max_batch=8, precision=precs[0])
+    srv.replay(arrivals(min(rates)), mats, rhss, keys, keeps)
+    srv.close()
+    lib.npph_prof_enable(0)
+    prof = prof_report(lib, ctypes)
+    lib.npph_prof_reset()
+    dom = max(prof, key=lambda k: prof[k]["ms"])
+    roof = roofline_from(prof, dom)
     cpu = None
     if not args.no_cpu:
         # The same trace served offline on one core, in arrival order, until
-        # about 15 s of service time: a reuse request solves with the kept
-        # factors of its owner (lu_solve), every other request factors.
+        # about --cpu-seconds of service time: a reuse request solves with the
+        # kept factors of its owner (lu_solve), every other request factors.
+        svc, kept = [], {}
         cpus = pin_one_core()
-        svc, kept, t_start = [], {}, time.perf_counter()
+        t_start = time.perf_counter()
         for i in range(R):
-            if time.perf_counter() - t_start > 15.0:
+            if time.perf_counter() - t_start > args.cpu_seconds:
                 break
             t1 = time.perf_counter()
             if mats[i] is None:
@@ -600,27 +659,41 @@ def bench_serve(args):
                 kept[i] = (lu, piv)
             svc.append((time.perf_counter() - t1) * 1e3)
         unpin(cpus)
-        model, ncpu = cpu_info()
-        cpu = {"value": len(svc) / (sum(svc) / 1e3), "unit": "solves/s", "cores": 1, "kind": "port",
-               "sample": f"the first {len(svc)} requests of the trace served one at a time in arrival order "
-                         f"(about 15 s of service; FP64 LU+solve, reuse via lu_solve; oracle C -O3 "
-                         f"-ffp-contract=off), 1 of {ncpu} cores ({model})",
-               "p50_ms": float(np.percentile(svc, 50)), "p99_ms": float(np.percentile(svc, 99))}
-    top = max(r["rate"] for r in runs)
-    head = [r for r in runs if r["rate"] == top and r["precision"] == "fp32"] or [r for r in runs if r["rate"] == top]
-    h = head[0]
-    plain = [r for r in runs if r["rate"] == top and r["precision"] == "fp64"]
+        cpu = cpu_record(len(svc), "solves/s", sum(svc) / 1e3, None,
+                         f"the first {len(svc)} requests of the trace served one at a time in arrival order "
+                         f"(about {args.cpu_seconds:g} s of service; FP64 LU+solve, reuse via lu_solve)",
+                         extra={"p50_ms": float(np.percentile(svc, 50)), "p99_ms": float(np.percentile(svc, 99))})
+    # headline: the highest rate the mixed-precision server sustains (completes
+    # at >= 90% of the offered rate), else the lowest rate
+    def sustained(prec):
+        rs = [r for r in runs if r["precision"] == prec]
+        good = [r for r in rs if r["solves_per_s"] >= 0.9 * r["rate"]]
+        return max(good, key=lambda r: r["rate"]) if good else min(rs, key=lambda r: r["rate"])
+    h = sustained(precs[0])
+    plain = [r for r in runs if r["precision"] == "fp64" and r["rate"] == h["rate"]]
     return {"metric": "solves/s + p99 latency, mixed-prec refined vs FP64 LU at backward err <=1e-12",
             "value": h["solves_per_s"], "unit": "solves/s", "n_gpus": 1, "steps": len(runs), "warmup": 1,
-            "ms_per_step": 1e3 * h["wall_s"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-            "dtype": "f32+f64" if h["precision"] == "fp32" else "f64", "data": "synthetic",
-            "config": {"workload": f"serving: {R} requests, Poisson rates {args.rates}/s, n log-U[16,4096], "
+            "ms_per_step": 1e3 * h["wall_s"] / R, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f32+f64" if h["precision"] == "fp32" else "f64", "data": "synthetic",
+            "config": {"workload": f"serving: {R} requests per rate over {min(R, args.distinct)} systems, "
+                                   f"Poisson rates {args.rates}/s, n log-U[16,4096], "
                                    f"nrhs log-U[1,64], kappa log-U[1e2,1e8], 30% reuse, budget 32e9 B, seed 5",
-                       "mode": h["precision"], "rate": top, "parallelism": "single-gpu"},
+                       "mode": h["precision"], "rate": h["rate"], "parallelism": "single-gpu"},
             "p50_ms": h["p50_ms"], "p99_ms": h["p99_ms"],
             "fp64_solves_per_s": plain[0]["solves_per_s"] if plain else None,
             "fp64_p99_ms": plain[0]["p99_ms"] if plain else None,
-            "runs": runs, "roofline": None, "cpu_baseline": cpu}
+            "knee": {p: sustained(p)["rate"] for p in precs},
+            "runs": runs, "roofline": roof, "kernel_ms_per_replay": {k: round(v["ms"], 3) for k, v in prof.items()},
+            "cpu_baseline": cpu}
+
+
+def slim(rec):
+    """A nested record: the fields of a bench line that describe one config."""
+    keep = ("metric", "value", "unit", "ms_per_step", "steps", "warmup", "higher_is_better", "dtype", "config",
+            "p50_ms", "p99_ms", "fp64_solves_per_s", "fp64_ms_per_step", "fp64_p99_ms", "speedup_vs_fp64",
+            "berr_max", "modes", "knee", "runs", "rel_err_vs_oracle", "roofline", "kernel_ms_per_step",
+            "kernel_ms_per_replay", "cpu_baseline", "wall_s")
+    return {k: v for k, v in rec.items() if k in keep}
 
 
 def main():
@@ -628,15 +701,18 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="mixed", choices=["mixed", "gemm", "plain64", "copy", "uneven", "serve"])
+    ap.add_argument("--workload", default="all",
+                    choices=["all", "mixed", "gemm", "plain64", "copy", "uneven", "serve"])
     ap.add_argument("--systems", type=int, default=20000, help="uneven workload: number of systems")
-    ap.add_argument("--requests", type=int, default=160, help="serve workload: requests per rate")
-    ap.add_argument("--rates", default="25,100,400", help="serve workload: Poisson rates, requests/s")
+    ap.add_argument("--requests", type=int, default=1000, help="serve workload: requests per rate")
+    ap.add_argument("--distinct", type=int, default=240, help="serve workload: distinct generated systems")
+    ap.add_argument("--rates", default="50,100,200,400", help="serve workload: Poisson rates, requests/s")
     ap.add_argument("--count", type=int, default=64)
     ap.add_argument("--n", type=int, default=2048)
-    ap.add_argument("--cpu-sample", type=int, default=16)
+    ap.add_argument("--cpu-sample", type=int, default=8)
+    ap.add_argument("--cpu-seconds", type=float, default=8.0, help="serve workload: offline CPU service budget")
     ap.add_argument("--modes", default="fp64,fp32,bf16x6,bf16x3",
-                    help="mixed workload: comma list of fp64, fp32, bf16x6, bf16x3")
+                    help="mixed: comma list of fp64, fp32, bf16x6, bf16x3 (uneven/serve use fp64 and fp32)")
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -647,8 +723,20 @@ def main():
         print("bench.py: multi-rank launch is not supported by this single-GPU build", file=sys.stderr)
         sys.exit(2)
     fn = {"mixed": bench_mixed, "gemm": bench_gemm, "plain64": bench_plain64, "copy": bench_copy,
-          "uneven": bench_uneven, "serve": bench_serve}[args.workload]
-    line = fn(args)
+          "uneven": bench_uneven, "serve": bench_serve}
+    if args.workload != "all":
+        line = fn[args.workload](args)
+    else:
+        t0 = time.time()
+        line = bench_mixed(args)
+        log(f"mixed done at {time.time() - t0:.0f}s")
+        nested = {}
+        for key, name in (("1_plain64", "plain64"), ("2_gemm", "gemm"), ("4_uneven", "uneven"), ("5_serve", "serve")):
+            t1 = time.time()
+            nested[key] = slim(fn[name](args))
+            nested[key]["wall_s"] = time.time() - t1
+            log(f"{name} done at {time.time() - t0:.0f}s")
+        line["configs"] = nested
     print(json.dumps(line, default=float))
 
 

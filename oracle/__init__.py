@@ -55,6 +55,12 @@ def _load():
     lib.orc_dot2_entry.restype = ctypes.c_double
     lib.orc_absdot_entry.argtypes = [ctypes.c_int64, ctypes.c_int64, _dp, _dp, ctypes.c_int64, ctypes.c_int64]
     lib.orc_absdot_entry.restype = ctypes.c_double
+    lib.orc_batched_solve_numpp.argtypes = [ctypes.c_int64, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp]
+    lib.orc_batched_solve_numpp.restype = ctypes.c_int64
+    lib.orc_matmul_numpp.argtypes = [ctypes.c_int64] * 3 + [_dp, _dp, _dp]
+    lib.orc_matmul_numpp.restype = None
+    lib.orc_nd_copy.argtypes = [_dp, ctypes.c_int64, ctypes.c_int64]
+    lib.orc_nd_copy.restype = ctypes.c_void_p
     lib.orc_det.argtypes = [_dp, ctypes.c_int]
     lib.orc_det.restype = ctypes.c_double
     lib.orc_slogdet.argtypes = [_dp, ctypes.c_int, _dp, _dp]
@@ -130,6 +136,26 @@ def batched_solve(a, b):
     info = np.zeros(count, np.int32)
     _load().orc_batched_solve(count, n, m, _ptr(a), _ptr(b), _ptr(x), _ptr(info, _ip))
     return x, info
+
+
+def batched_solve_numpp(a, b):
+    """batched.cpp:90-100 with NumPP's per-matrix ndarray copies (the
+    "NumPP-equivalent" CPU time). Returns X (same values as batched_solve)."""
+    a = _c(a)
+    b = _c(b)
+    count, n = a.shape[0], a.shape[1]
+    m = 1 if b.ndim == a.ndim - 1 else b.shape[-1]
+    x = np.empty_like(b)
+    _load().orc_batched_solve_numpp(count, n, m, _ptr(a), _ptr(b), _ptr(x))
+    return x
+
+
+def matmul_numpp(a, b):
+    """backend.cpp:55-84 with the astype copies of both operands (float64)."""
+    a, b = _c(a), _c(b)
+    c = np.empty((a.shape[0], b.shape[1]))
+    _load().orc_matmul_numpp(a.shape[0], b.shape[1], a.shape[1], _ptr(a), _ptr(b), _ptr(c))
+    return c
 
 
 def det(a):

@@ -17,6 +17,12 @@
  *   orc_det            <- src/linalg/linalg.cpp:393-407 (det)
  *   orc_slogdet        <- src/linalg/linalg.cpp:408-439 (slogdet, real case)
  *   orc_inv            <- src/linalg/linalg.cpp:105-115 (inv_impl)
+ *   orc_nd_copy        <- src/core/ndarray.cpp:216-231, 257-285 (copy / astype:
+ *                         element-by-element multi-index loop), for the
+ *                         "NumPP-equivalent" CPU timings below
+ *   orc_batched_solve_numpp <- batched.cpp:90-100 with the ndarray copies each
+ *                         matrix goes through (copy(), to_vec, from_vec, stack)
+ *   orc_matmul_numpp   <- backend.cpp:55-84 with its two astype copies
  * Helpers that NumPP does not have (pinned by mathematics, not by NumPP):
  *   orc_dot2_entry     compensated dot product (Ogita-Rump-Oishi Dot2) of one
  *                      GEMM entry, used to bound GEMM error at sizes where the
@@ -196,6 +202,61 @@ int64_t orc_batched_solve(int64_t count, int n, int m, const double* A, const do
   return first;
 }
 
+/* ndarray.cpp:216-231 (copy) and 257-285 (astype) for a 2-D float64 array:
+ * a fresh buffer, then one element at a time through the multi-index loop
+ * (offsets recomputed from the index vector and the strides per element). */
+double* orc_nd_copy(const double* src, int64_t rows, int64_t cols) {
+  const int64_t shape[2] = {rows, cols};
+  const int64_t strides[2] = {cols * 8, 8};
+  double* out = (double*)malloc(sizeof(double) * (size_t)(rows * cols > 0 ? rows * cols : 1));
+  if (rows * cols == 0) return out;
+  int64_t idx[2] = {0, 0};
+  const char* sb = (const char*)src;
+  char* db = (char*)out;
+  for (;;) {
+    int64_t soff = 0, doff = 0;
+    for (int i = 0; i < 2; ++i) { soff += idx[i] * strides[i]; doff += idx[i] * strides[i]; }
+    memcpy(db + doff, sb + soff, 8);
+    int ax = 1;
+    for (; ax >= 0; --ax) { if (++idx[ax] < shape[ax]) break; idx[ax] = 0; }
+    if (ax < 0) break;
+  }
+  return out;
+}
+
+/* batched.cpp:90-100 as NumPP executes it, copies included: per matrix,
+ * s.a3[i].copy() and b3[i].copy() (ndarray copy), to_vec (astype + vector
+ * construction, linalg.cpp:35-40), the solve, from_vec (buffer memcpy + astype,
+ * linalg.cpp:41-46), and stack's astype + memcpy into the output
+ * (batched.cpp:34-48). X receives the solutions; returns as orc_batched_solve. */
+int64_t orc_batched_solve_numpp(int64_t count, int n, int m, const double* A, const double* B, double* X) {
+  int64_t first = -1;
+  const int64_t as = (int64_t)n * n, bs = (int64_t)n * m;
+  int* piv = (int*)malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
+  for (int64_t i = 0; i < count; ++i) {
+    double* ac = orc_nd_copy(A + i * as, n, n);       /* s.a3[i].copy() */
+    double* bc = orc_nd_copy(B + i * bs, n, m);       /* b3[i].copy() */
+    double* at = orc_nd_copy(ac, n, n);               /* to_vec: astype */
+    double* bt = orc_nd_copy(bc, n, m);
+    double* av = (double*)malloc(sizeof(double) * (size_t)(as > 0 ? as : 1));  /* std::vector(p, p + size) */
+    double* bv = (double*)malloc(sizeof(double) * (size_t)(bs > 0 ? bs : 1));
+    memcpy(av, at, sizeof(double) * (size_t)as);
+    memcpy(bv, bt, sizeof(double) * (size_t)bs);
+    double sgn;
+    const int s = orc_lu_decompose(av, n, piv, NULL, &sgn);
+    if (s < 0) orc_lu_solve(av, n, piv, bv, m);
+    else if (first < 0) first = i;
+    double* r = (double*)malloc(sizeof(double) * (size_t)(bs > 0 ? bs : 1));  /* from_vec */
+    memcpy(r, bv, sizeof(double) * (size_t)bs);
+    double* ro = orc_nd_copy(r, n, m);                /* from_vec: astype(out) */
+    double* rs = orc_nd_copy(ro, n, m);               /* stack: astype(cd) */
+    memcpy(X + i * bs, rs, sizeof(double) * (size_t)bs);
+    free(ac); free(bc); free(at); free(bt); free(av); free(bv); free(r); free(ro); free(rs);
+  }
+  free(piv);
+  return first;
+}
+
 /* backend.cpp:55-84 for float64: zero the row, then i-p-j accumulation. */
 void orc_matmul(int64_t m, int64_t n, int64_t k, const double* A, const double* B, double* C) {
   for (int64_t i = 0; i < m; ++i) {
@@ -207,6 +268,15 @@ void orc_matmul(int64_t m, int64_t n, int64_t k, const double* A, const double* 
       for (int64_t j = 0; j < n; ++j) crow[j] += aip * brow[j];
     }
   }
+}
+
+/* backend.cpp:55-84 as NumPP executes it for float64: the astype copies of
+ * both operands (lines 59-60), then the i-p-j loop. */
+void orc_matmul_numpp(int64_t m, int64_t n, int64_t k, const double* A, const double* B, double* C) {
+  double* ac = orc_nd_copy(A, m, k);
+  double* bc = orc_nd_copy(B, k, n);
+  orc_matmul(m, n, k, ac, bc, C);
+  free(ac); free(bc);
 }
 
 /* backend.cpp:55-84 for float32 (compute dtype stays float32). */

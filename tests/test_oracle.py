@@ -307,3 +307,14 @@ def test_det_special_cases():
     assert oracle.det(a) == 0.0 and oracle.slogdet(a) == (0.0, -np.inf)
     with pytest.raises(oracle.SingularMatrix):
         oracle.inv(a)
+
+
+def test_numpp_equivalent_timing_paths_compute_the_same():
+    # the "NumPP-equivalent" CPU baselines add NumPP's ndarray copies
+    # (ndarray.cpp:216-231, 257-285) around the same kernels: same values
+    a, b = config1_inputs()
+    x1, _ = oracle.batched_solve(a[:8], b[:8])
+    np.testing.assert_array_equal(oracle.batched_solve_numpp(a[:8], b[:8]), x1)
+    rng = np.random.default_rng(13)
+    A, B = rng.standard_normal((30, 20)), rng.standard_normal((20, 10))
+    np.testing.assert_array_equal(oracle.matmul_numpp(A, B), oracle.matmul(A, B))
