@@ -1,19 +1,32 @@
-// Batched LU with partial pivoting + getrs for n <= 64: one wavefront per
+// Batched LU with partial pivoting + getrs for n <= 64: one workgroup per
 // matrix, the matrix resident in registers.
 //
-// Lane r owns original row r (N doubles, N = n rounded up to 8/16/32/64; the
-// padding is the identity, which leaves the leading n x n factorisation, its
-// pivots and its exact-zero singularity test unchanged). Rows are never moved
-// between lanes: each lane tracks the logical position `pos` of its row, and a
-// step-k swap of positions k and p (linalg.cpp:58-62) just exchanges two
-// lanes' `pos` values. The pivot search is a wave-wide argmax over
-// (|a[k]|, pos) with NumPP's tie rule (largest magnitude, then smallest current
-// position: linalg.cpp:55-56; a NaN at position k is kept, a NaN below it is
-// never chosen), the pivot row is broadcast with v_readlane into
-// scalar registers, and each lane applies the rank-1 update to its own row
-// (linalg.cpp:64-68, division by the pivot as in the reference). The solve
-// walks the same lanes: forward substitution with the unit-lower L, then back
-// substitution dividing by U's diagonal (linalg.cpp:73-90).
+// The matrix is padded to N = 8, 16, 32 or 64 with the identity (which leaves
+// the leading n x n factorisation, its pivots and its exact-zero singularity
+// test unchanged) and split by columns over NW = N/16 waves (one wave for
+// N <= 16): lane r of every wave holds original row r, wave w the columns
+// [w CPW, (w+1) CPW). Rows are never moved: each wave tracks the logical
+// position `pos` of every lane's row, and the step-k swap of positions k and p
+// (linalg.cpp:58-62) exchanges two lanes' `pos` values.
+//
+// The k loop is unrolled at compile time, so column k is a static register and
+// every wave knows statically whether it owns column k and which of its columns
+// lie right of k. Step k:
+//   - the owner wave of column k finds the pivot with a wave-wide argmax over
+//     (|a_k|, pos) with NumPP's rule (largest magnitude, then smallest current
+//     position, linalg.cpp:55-56; a NaN at position k is kept, a NaN below it is
+//     never chosen), divides the column by the pivot (linalg.cpp:65-66) and
+//     publishes the multipliers, the pivot position and the pivot lane in LDS;
+//   - one workgroup barrier (the LDS slots alternate between even and odd
+//     steps, so the next owner never overwrites what a slower wave still reads);
+//   - every wave with columns right of k updates them (linalg.cpp:67), reading
+//     the pivot row from its own lane `wl` with v_readlane.
+// The factors go to HBM (row order = logical position) and to LDS, where the
+// solve reads them: forward substitution with the unit-lower L, then back
+// substitution with U (linalg.cpp:73-90), lane = logical row, waves taking
+// right-hand-side chunks in turn.
+#include <utility>
+
 #include "common.hpp"
 #include "lu.hpp"
 #include "prof.hpp"
@@ -25,127 +38,213 @@ namespace {
 constexpr int NR = 8;  // right-hand-side columns held in registers per pass
 
 template <int N>
-__device__ void lu_wave(const SmallSys& s) {
-  const int lane = threadIdx.x & 63;
-  const int n = s.n;
-  const bool act = lane < N;
-  double a[N];
-  const double* Ar = s.A + (int64_t)lane * s.lda;
-#pragma unroll
-  for (int j = 0; j < N; ++j) a[j] = (lane < n && j < n) ? Ar[j] : (lane == j ? 1.0 : 0.0);
+struct Small {
+  static constexpr int NW = N >= 32 ? N / 16 : 1;  // waves per matrix
+  static constexpr int CPW = N / NW;               // columns per wave
+  static constexpr int NT = 64 * NW;
+};
 
-  int pos = act ? lane : 1 << 20;
-  int sing = -1;
-  // plane[k]: lane holding logical row k (kept in LDS so the solve can index it).
-  __shared__ int plane[64];
+// Shared state of one workgroup.
+template <int N>
+struct SmallShared {
+  double l[2][64];       // multipliers of the step (by lane), even / odd steps
+  int p[2], wl[2];       // pivot position and pivot lane of the step
+  int sing;              // first step with an exact-zero pivot, or -1
+  double L[N][N + 1];    // strictly lower factor by logical row (zeros elsewhere)
+  double U[N][N + 1];    // strictly upper factor by logical row, column k / u_kk
+  double D[N];           // diagonal of U
+  int perm[N];           // original row at each logical position
+};
 
-  for (int k = 0; k < N; ++k) {
-    double ak = a[0];
-#pragma unroll
-    for (int j = 1; j < N; ++j) ak = (j == k) ? a[j] : ak;
-    double v = (act && pos >= k) ? piv_key(ak, pos, k) : -1.0;
-    int p = pos;
-    wave_argmax(v, p);
-    p = __builtin_amdgcn_readfirstlane(p);
-    if (v == 0.0 && k < n) { sing = k; break; }
-    const uint64_t m = __ballot(pos == p);
-    const int wl = __builtin_amdgcn_readfirstlane((int)__builtin_ctzll(m));
-    if (s.ipiv && lane == 0 && k < n) s.ipiv[k] = p;
-    if (lane == 0) plane[k] = wl;
-    const bool is_k = (pos == k);
-    if (lane == wl) pos = k;
-    else if (is_k) pos = p;
-    const double piv = readlane_d(ak, wl);
-    const double l = (pos > k) ? ak / piv : 0.0;
-#pragma unroll
-    for (int j = 0; j < N; ++j) {
-      double u = readlane_d(a[j], wl);
-      u = (j > k) ? u : 0.0;
-      a[j] = fma(-l, u, a[j]);
-      if (j == k && pos > k) a[j] = l;
-    }
-  }
-  __syncthreads();  // one-wave block: orders lane 0's plane[] writes before the solve reads
-  if (s.info && lane == 0) *s.info = sing < 0 ? 0 : sing + 1;
-  if (s.LU && lane < n && pos < n) {
-    double* o = s.LU + (int64_t)pos * s.ldlu;
-#pragma unroll
-    for (int j = 0; j < N; ++j)
-      if (j < n) o[j] = a[j];
-  }
-  if (sing >= 0 || s.nrhs <= 0 || !s.X) return;
-
-  for (int c0 = 0; c0 < s.nrhs; c0 += NR) {
-    double b[NR];
-    const double* Br = s.B + (int64_t)lane * s.ldb + c0;
-#pragma unroll
-    for (int q = 0; q < NR; ++q) b[q] = (lane < n && c0 + q < s.nrhs) ? Br[q] : 0.0;
-    // forward: unit-lower L, logical row order 0..N-1
-    for (int i = 0; i < N; ++i) {
-      const int wl = plane[i];
-      double li = a[0];
-#pragma unroll
-      for (int j = 1; j < N; ++j) li = (j == i) ? a[j] : li;
-      li = (pos > i) ? li : 0.0;
-#pragma unroll
-      for (int q = 0; q < NR; ++q) b[q] = fma(-li, readlane_d(b[q], wl), b[q]);
-    }
-    // back: upper U with division by the diagonal, rows N-1..0
-    for (int i = N - 1; i >= 0; --i) {
-      const int wl = plane[i];
-      double ui = a[0];
-#pragma unroll
-      for (int j = 1; j < N; ++j) ui = (j == i) ? a[j] : ui;
-      if (pos == i) {
-#pragma unroll
-        for (int q = 0; q < NR; ++q) b[q] = b[q] / ui;
+// Factorisation, as seen by wave W (all arguments uniform across the wave).
+template <int N, int W>
+__device__ __forceinline__ void factor(SmallShared<N>& sh, double (&a)[Small<N>::CPW], int& pos, int n, int lane,
+                                       int32_t* ipiv) {
+  constexpr int CPW = Small<N>::CPW;
+  bool stop = false;
+  auto step = [&]<int K>() __attribute__((always_inline)) {
+    if (stop || K >= n) return;  // uniform over the workgroup: n and sing are
+    constexpr int O = K / CPW, CO = K % CPW, BUF = K & 1;
+    double l = 0.0;
+    int wl = 0, p = K;
+    if constexpr (W == O) {
+      double v = (pos >= K && pos < N) ? piv_key(a[CO], pos, K) : -1.0;
+      p = pos;
+      wave_argmax(v, p);
+      p = __builtin_amdgcn_readfirstlane(p);
+      const uint64_t m = __ballot(pos == p);
+      wl = __builtin_amdgcn_readfirstlane((int)__builtin_ctzll(m));
+      const double piv = readlane_d(a[CO], wl);
+      if (pos == p) pos = K;
+      else if (pos == K) pos = p;
+      if (pos > K && pos < N) {
+        l = a[CO] / piv;
+        a[CO] = l;
       }
-      const double f = (pos < i) ? ui : 0.0;
-#pragma unroll
-      for (int q = 0; q < NR; ++q) b[q] = fma(-f, readlane_d(b[q], wl), b[q]);
+      sh.l[BUF][lane] = l;
+      if (lane == 0) {
+        sh.p[BUF] = p;
+        sh.wl[BUF] = wl;
+        if (ipiv) ipiv[K] = p;
+        if (v == 0.0) sh.sing = K;
+      }
     }
-    if (lane < n && pos < n) {
-      double* Xr = s.X + (int64_t)pos * s.ldx + c0;
+    __syncthreads();
+    if constexpr (W != O) {
+      p = sh.p[BUF];
+      wl = sh.wl[BUF];
+      if (pos == p) pos = K;
+      else if (pos == K) pos = p;
+      l = sh.l[BUF][lane];
+    }
+    if (sh.sing >= 0) {
+      stop = true;
+      return;
+    }
+    // columns of this wave right of the pivot column
+#pragma unroll
+    for (int c = 0; c < CPW; ++c) {
+      if (W * CPW + c > K) {
+        const double u = readlane_d(a[c], wl);
+        a[c] = fma(-l, u, a[c]);
+      }
+    }
+  };
+  [&]<int... K>(std::integer_sequence<int, K...>) __attribute__((always_inline)) {
+    (step.template operator()<K>(), ...);
+  }(std::make_integer_sequence<int, N>{});
+}
+
+template <int N, int W>
+__device__ __forceinline__ void wave_body(const UniformBatch& u, SmallShared<N>& sh, int64_t mat, int lane) {
+  constexpr int CPW = Small<N>::CPW, NW = Small<N>::NW;
+  const int n = u.n;
+  double a[CPW];
+  {
+    const double* Ar = u.A + mat * u.strideA + (int64_t)(lane < n ? lane : 0) * n;
+#pragma unroll
+    for (int c = 0; c < CPW; ++c) {
+      const int j = W * CPW + c;
+      const double v = Ar[j < n ? j : 0];
+      a[c] = (lane < n && j < n) ? v : (lane == j ? 1.0 : 0.0);
+    }
+  }
+  int pos = lane < N ? lane : (1 << 20);
+  factor<N, W>(sh, a, pos, n, lane, u.ipiv ? u.ipiv + mat * n : nullptr);
+  const bool singular = sh.sing >= 0;
+  if (W == 0 && lane == 0 && u.info) u.info[mat] = singular ? sh.sing + 1 : 0;
+  if (u.LU && lane < n && pos < n) {
+    double* o = u.LU + mat * u.strideA + (int64_t)pos * n;
+#pragma unroll
+    for (int c = 0; c < CPW; ++c)
+      if (W * CPW + c < n) o[W * CPW + c] = a[c];
+  }
+  if (singular || u.nrhs <= 0 || !u.X) return;  // uniform over the workgroup
+  // Solve operands in LDS by logical row, shaped so the per-step reads need no
+  // lane masks: L strictly lower (zeros elsewhere) and V = U D^-1 strictly
+  // upper (column k of U divided by u_kk). Back substitution then runs on
+  // z = D x: z_i -= V[i][k] z_k, and x_i = z_i / u_ii at the end. (linalg.cpp:
+  // 83-88 divides x_k by u_kk before it is used; dividing U's column instead
+  // gives the same solution within rounding.)
+  if (lane < N) {
+#pragma unroll
+    for (int c = 0; c < CPW; ++c)
+      if (W * CPW + c == pos) sh.D[pos] = a[c];
+    if (W == 0) sh.perm[pos] = lane;
+  }
+  __syncthreads();
+  if (lane < N) {
+#pragma unroll
+    for (int c = 0; c < CPW; ++c) {
+      const int j = W * CPW + c;
+      sh.L[pos][j] = j < pos ? a[c] : 0.0;
+      sh.U[pos][j] = j > pos ? a[c] / sh.D[j] : 0.0;
+    }
+  }
+  __syncthreads();
+  const double* B = u.B + mat * u.strideB;
+  double* X = u.X + mat * u.strideB;
+  const int nrhs = u.nrhs;
+  const int src = lane < n ? sh.perm[lane] : 0;
+  const int row = lane < N ? lane : 0;
+  const double dl = sh.D[row];
+  // waves take chunks of NR right-hand-side columns in turn
+  for (int c0 = W * NR; c0 < nrhs; c0 += NW * NR) {
+    // n through an opaque per-chunk zero: the 2N step guards below are then
+    // not hoisted out of the chunk loop as 2N saved masks
+    int z;
+    asm volatile("s_mov_b32 %0, 0" : "=s"(z));
+    const int nn = n + z;
+    double y[NR];
+#pragma unroll
+    for (int q = 0; q < NR; ++q) y[q] = (lane < n && c0 + q < nrhs) ? B[(int64_t)src * nrhs + c0 + q] : 0.0;
+    // forward: y_i -= L[i][k] y_k (row order = logical position)
+#pragma unroll
+    for (int k = 0; k < N - 1; ++k) {
+      if (k + 1 >= nn) break;
+      const double f = sh.L[row][k];
+#pragma unroll
+      for (int q = 0; q < NR; ++q) y[q] = fma(-f, readlane_d(y[q], k), y[q]);
+    }
+    // back: z_i -= V[i][k] z_k for i < k
+#pragma unroll
+    for (int k = N - 1; k >= 1; --k) {
+      if (k >= nn) continue;
+      const double f = sh.U[row][k];
+#pragma unroll
+      for (int q = 0; q < NR; ++q) y[q] = fma(-f, readlane_d(y[q], k), y[q]);
+    }
+    if (lane < n) {
+      double* xr = X + (int64_t)lane * nrhs + c0;
 #pragma unroll
       for (int q = 0; q < NR; ++q)
-        if (c0 + q < s.nrhs) Xr[q] = b[q];
+        if (c0 + q < nrhs) xr[q] = y[q] / dl;
     }
   }
 }
 
-// Uniform stack: block = 1 wave, blockIdx.x = matrix.
+// Uniform stack: one workgroup of NW waves per matrix.
 template <int N>
-__global__ __launch_bounds__(64) void lu_wave_uniform(UniformBatch u) {
-  const int64_t i = blockIdx.x;
-  SmallSys s;
-  s.n = u.n;
-  s.A = u.A + i * u.strideA;
-  s.lda = u.n;
-  s.LU = u.LU ? u.LU + i * u.strideA : nullptr;
-  s.ldlu = u.n;
-  s.ipiv = u.ipiv ? u.ipiv + i * u.n : nullptr;
-  s.info = u.info ? u.info + i : nullptr;
-  s.nrhs = u.nrhs;
-  s.B = u.B ? u.B + i * u.strideB : nullptr;
-  s.ldb = u.nrhs;
-  s.X = u.X ? u.X + i * u.strideB : nullptr;
-  s.ldx = u.nrhs;
-  lu_wave<N>(s);
+__global__ __launch_bounds__(Small<N>::NT) void lu_small_kernel(UniformBatch u) {
+  __shared__ SmallShared<N> sh;
+  const int64_t mat = blockIdx.x;
+  // wave index made provably uniform, so the per-wave bodies are scalar
+  // branches (no exec-mask state kept live across them)
+  const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  if (threadIdx.x == 0) sh.sing = -1;
+  __syncthreads();
+  if constexpr (Small<N>::NW == 1) {
+    wave_body<N, 0>(u, sh, mat, lane);
+  } else if constexpr (Small<N>::NW == 2) {
+    if (wave == 0) wave_body<N, 0>(u, sh, mat, lane);
+    else wave_body<N, 1>(u, sh, mat, lane);
+  } else {
+    switch (wave) {
+      case 0: wave_body<N, 0>(u, sh, mat, lane); break;
+      case 1: wave_body<N, 1>(u, sh, mat, lane); break;
+      case 2: wave_body<N, 2>(u, sh, mat, lane); break;
+      default: wave_body<N, 3>(u, sh, mat, lane); break;
+    }
+  }
 }
 
 }  // namespace
 
 int small_lu_launch(const UniformBatch& u, hipStream_t st) {
   if (u.count <= 0 || u.n <= 0) return NPPH_OK;
-  dim3 grid((unsigned)u.count), block(64);
   if (u.count > 0x7fffffff) { set_error("batch too large"); return NPPH_ERR_INVALID; }
+  if (u.nrhs > 0 && u.X && u.B != u.X) {
+    set_error("small_lu: the solve works in place (X must alias B)");
+    return NPPH_ERR_INVALID;
+  }
+  dim3 grid((unsigned)u.count);
   const double n = u.n, flu = n * (n - 1) / 2 + (n - 1) * n * (2 * n - 1) / 3;
   ProfScope ps(kProfSmallLu, u.count * (flu + (2 * n * n - n) * u.nrhs),
                8.0 * u.count * (n * n + 2 * n * u.nrhs + (u.LU ? n * n : 0)), st);
-  if (u.n <= 8) hipLaunchKernelGGL(lu_wave_uniform<8>, grid, block, 0, st, u);
-  else if (u.n <= 16) hipLaunchKernelGGL(lu_wave_uniform<16>, grid, block, 0, st, u);
-  else if (u.n <= 32) hipLaunchKernelGGL(lu_wave_uniform<32>, grid, block, 0, st, u);
-  else if (u.n <= 64) hipLaunchKernelGGL(lu_wave_uniform<64>, grid, block, 0, st, u);
+  if (u.n <= 8) hipLaunchKernelGGL(lu_small_kernel<8>, grid, dim3(Small<8>::NT), 0, st, u);
+  else if (u.n <= 16) hipLaunchKernelGGL(lu_small_kernel<16>, grid, dim3(Small<16>::NT), 0, st, u);
+  else if (u.n <= 32) hipLaunchKernelGGL(lu_small_kernel<32>, grid, dim3(Small<32>::NT), 0, st, u);
+  else if (u.n <= 64) hipLaunchKernelGGL(lu_small_kernel<64>, grid, dim3(Small<64>::NT), 0, st, u);
   else { set_error("small_lu: n=%d > 64", u.n); return NPPH_ERR_INVALID; }
   NPPH_HIP(hipGetLastError());
   return NPPH_OK;
