@@ -124,6 +124,21 @@ int npph_getrs_batched_dev(int64_t count, int64_t n, int64_t nrhs, const double*
 int npph_gesv_batched_dev(int64_t count, int64_t n, int64_t nrhs, double* A, double* B,
                           int32_t* ipiv, int32_t* info, void* stream);
 
+/* Allocation-free form of npph_gesv_batched_dev, for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). No counterpart in the reference. Same
+ * arguments and results as npph_gesv_batched_dev; its scratch (pivots and info
+ * when passed NULL, the permutation, row-move lists and staging, solve staging)
+ * comes from a fixed 64 MiB device array per GPU instead of library
+ * allocations, and it calls no allocating routine. Calls whose scratch would
+ * exceed 64 MiB fail with NPPH_ERR_INVALID before any work is issued (n <= 64
+ * needs at most the pivots and info; n = 2048 with 8 right-hand sides needs
+ * about 18 MB per matrix). Every call on one GPU shares that array: calls that
+ * overlap on one GPU (different streams or threads) race and must be ordered
+ * by the caller. */
+int npph_gesv_batched_dev_noalloc(int64_t count, int64_t n, int64_t nrhs, double* A, double* B,
+                                  int32_t* ipiv, int32_t* info, void* stream);
+
 /* ---------------- determinants and inverses ---------------- */
 /* New GpuVTable slots det_batched / slogdet_batched: replace the per-matrix
  * loops of batched::det and batched::slogdet (src/linalg/batched.cpp:76,

@@ -96,6 +96,7 @@ SIGNATURES = {
     "npph_getrf_batched_dev": (_int, [_i64, _i64, _vp, _i64, _vp, _vp, _vp]),
     "npph_getrs_batched_dev": (_int, [_i64, _i64, _i64, _vp, _i64, _vp, _vp, _i64, _vp]),
     "npph_gesv_batched_dev": (_int, [_i64, _i64, _i64, _vp, _vp, _vp, _vp, _vp]),
+    "npph_gesv_batched_dev_noalloc": (_int, [_i64, _i64, _i64, _vp, _vp, _vp, _vp, _vp]),
     "npph_det_batched": (_int, [_int, _i64, _i64, _vp, _vp, _vp, _vp, _vp]),
     "npph_inv_batched": (_int, [_int, _i64, _i64, _vp, _vp, _vp]),
     "npph_det_batched_dev": (_int, [_i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp]),

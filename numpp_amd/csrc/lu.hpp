@@ -41,6 +41,9 @@ int small_lu_launch(const UniformBatch& u, hipStream_t s);
 
 // Largest order the blocked LU supports (panel rows held in registers).
 int max_blocked_n();
+// Bytes of workspace one blocked factorisation of count matrices of order n
+// with elem-byte entries requests, each request rounded up to 256 bytes.
+size_t blocked_getrf_scratch_bytes(int64_t count, int n, size_t elem);
 
 // Blocked right-looking LU in working precision T (double or float) of a
 // uniform stack stored in place (count x n x n, element stride strideA).

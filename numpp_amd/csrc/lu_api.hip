@@ -3,6 +3,7 @@
 // Size dispatch: n <= 64 runs the one-wave-per-matrix kernel (factor and
 // solve in one launch, matrix in registers); larger n runs the blocked
 // right-looking LU with MFMA trailing updates and the blocked getrs.
+#include <mutex>
 #include <vector>
 
 #include "common.hpp"
@@ -40,6 +41,51 @@ int gesv_dev_impl(int64_t count, int n, int nrhs, double* A, int64_t strideA, do
   }
   return NPPH_OK;
 }
+
+// Scratch bytes gesv_dev_impl asks for (each request rounded up to 256 bytes,
+// as a ScratchArena carves them): pivots and info when the caller passes none,
+// and for n > 64 the permutation, the factorisation's workspaces and getrs'
+// output staging.
+size_t gesv_scratch_bytes(int64_t count, int n, int nrhs, bool has_ipiv, bool has_info) {
+  auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  size_t t = 0;
+  if (!has_ipiv) t += al(sizeof(int32_t) * (size_t)count * n);
+  if (!has_info) t += al(sizeof(int32_t) * (size_t)count);
+  if (n <= 64) return t;
+  t += al(sizeof(int32_t) * (size_t)count * n);
+  t += blocked_getrf_scratch_bytes(count, n, sizeof(double));
+  if (nrhs > 0) t += al(sizeof(double) * (size_t)count * n * nrhs);
+  return t;
+}
+
+namespace {
+
+// Device-resident scratch of the allocation-free entry point: one copy per GPU,
+// held until the process ends. Host code never uses the array's name as an
+// address: each device's copy is looked up once with hipGetSymbolAddress and
+// that pointer is what kernels receive.
+constexpr size_t kScratchBytes = 64ull << 20;
+static __device__ __align__(256) char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+int scratch_ptr(void** out) {
+  int dev = 0;
+  NPPH_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) { set_error("no-alloc gesv: device %d out of range", dev); return NPPH_ERR_INVALID; }
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    NPPH_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return NPPH_OK;
+}
+
+}  // namespace
 
 }  // namespace npph
 
@@ -98,6 +144,33 @@ int npph_gesv_batched_dev(int64_t count, int64_t n, int64_t nrhs, double* A, dou
   const hipStream_t s = pick(stream);
   CallScope cs(s);
   return gesv_dev_impl(count, (int)n, (int)nrhs, A, n * n, B, n * nrhs, ipiv, info, s);
+}
+
+int npph_gesv_batched_dev_noalloc(int64_t count, int64_t n, int64_t nrhs, double* A, double* B, int32_t* ipiv,
+                                  int32_t* info, void* stream) {
+  NPPH_TRY(ensure_init());
+  if (count < 0 || n < 0 || nrhs < 0 || n > max_blocked_n()) {
+    set_error("no-alloc gesv: bad sizes (count=%ld n=%ld nrhs=%ld)", (long)count, (long)n, (long)nrhs);
+    return NPPH_ERR_INVALID;
+  }
+  if (count == 0 || n == 0) return NPPH_OK;
+  const size_t need = gesv_scratch_bytes(count, (int)n, (int)nrhs, ipiv != nullptr, info != nullptr);
+  if (need > kScratchBytes) {
+    set_error("no-alloc gesv: needs %zu bytes of scratch for count=%ld n=%ld nrhs=%ld, above the %zu-byte maximum",
+              need, (long)count, (long)n, (long)nrhs, kScratchBytes);
+    return NPPH_ERR_INVALID;
+  }
+  void* base;
+  NPPH_TRY(scratch_ptr(&base));
+  ScratchArena arena;
+  arena.base = static_cast<char*>(base);
+  arena.bytes = kScratchBytes;
+  arena_bind(&arena);
+  // with the arena bound, workspace() carves from it and large
+  // factorisations are issued as plain launches (no graph instantiation)
+  const int st = gesv_dev_impl(count, (int)n, (int)nrhs, A, n * n, B, n * nrhs, ipiv, info, pick(stream));
+  arena_bind(nullptr);
+  return st;
 }
 
 int npph_gesv_batched(int dtype, int64_t count, int64_t n, int64_t nrhs, void* A, void* B, int32_t* ipiv,

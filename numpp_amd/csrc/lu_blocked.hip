@@ -322,6 +322,20 @@ struct Rec {
 };
 
 
+// Row-move staging per matrix: the largest row move is at the top level,
+// min(n, 2 w1) rows of w2 columns with w1 >= n/2, at most n * n/2 elements.
+int64_t getrf_move_stride(int n) { return (int64_t)n * (n / 2 + LEAF); }
+
+// Workspace bytes one factorisation asks for (leaf move lists, composed move
+// lists, row-move staging).
+struct GetrfWs {
+  size_t leafmv, net, move;
+};
+GetrfWs getrf_ws(int64_t count, int n, size_t elem) {
+  return {sizeof(int32_t) * (size_t)count * (size_t)ceil_div(n, LEAF) * LSTRIDE,
+          sizeof(int32_t) * (size_t)count * (1 + 2 * (size_t)n), elem * (size_t)count * getrf_move_stride(n)};
+}
+
 template <class T>
 int getrf_group(int64_t count, int n, T* A, int64_t strideA, int32_t* ipiv, int32_t* perm, int32_t* info,
                 int bf16_terms, const int32_t* mask, hipStream_t s) {
@@ -337,12 +351,11 @@ int getrf_group(int64_t count, int n, T* A, int64_t strideA, int32_t* ipiv, int3
   r.bf16_terms = bf16_terms;
   r.s = s;
   r.nleaf = (int)ceil_div(n, LEAF);
-  // The largest row move is at the top level: min(n, 2 w1) rows of w2 columns
-  // with w1 >= n/2, at most n * n/2 elements.
-  r.scr_stride = (int64_t)n * (n / 2 + LEAF);
-  NPPH_TRY(workspace(kWsMisc, sizeof(int32_t) * (size_t)count * r.nleaf * LSTRIDE, (void**)&r.leafmv));
-  NPPH_TRY(workspace(kWsNet, sizeof(int32_t) * (size_t)count * (1 + 2 * (size_t)n), (void**)&r.net));
-  NPPH_TRY(workspace(kWsMove, sizeof(T) * (size_t)count * r.scr_stride, (void**)&r.scratch));
+  r.scr_stride = getrf_move_stride(n);
+  const GetrfWs ws = getrf_ws(count, n, sizeof(T));
+  NPPH_TRY(workspace(kWsMisc, ws.leafmv, (void**)&r.leafmv));
+  NPPH_TRY(workspace(kWsNet, ws.net, (void**)&r.net));
+  NPPH_TRY(workspace(kWsMove, ws.move, (void**)&r.scratch));
   {
     const int64_t tot = count * n;
     hipLaunchKernelGGL(init_factor_kernel, dim3((unsigned)ceil_div(tot, 256)), dim3(256), 0, s, count, n, perm, info);
@@ -354,6 +367,12 @@ int getrf_group(int64_t count, int n, T* A, int64_t strideA, int32_t* ipiv, int3
 }  // namespace
 
 int max_blocked_n() { return 8 * PT; }
+
+size_t blocked_getrf_scratch_bytes(int64_t count, int n, size_t elem) {
+  const GetrfWs w = getrf_ws(count, n, elem);
+  auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  return al(w.leafmv) + al(w.net) + al(w.move);
+}
 
 namespace {
 

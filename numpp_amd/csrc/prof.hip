@@ -33,6 +33,7 @@ bool prof_enabled() { return g_on.load(std::memory_order_relaxed) != 0; }
 
 void prof_begin(int kind, double flops, double bytes, hipStream_t s, void** token) {
   if (stream_capturing(s)) return;  // no event records inside a captured graph
+  if (arena_bound()) return;         // allocation-free calls create no events
   std::lock_guard<std::mutex> g(g_mu);
   Rec r{kind, flops, bytes, take_event(), take_event()};
   if (!r.a || !r.b) return;
