@@ -36,6 +36,7 @@ extern "C" {
 #define NPPH_ERR_UNSUPPORTED (-5)
 #define NPPH_ERR_BUDGET (-6)
 #define NPPH_ERR_NOT_FOUND (-7)
+#define NPPH_ERR_SINGULAR (-8)
 
 /* dtype ids: the values of numpp::DTypeId (include/numpp/core/dtype.hpp:16-24). */
 #define NPPH_DT_F32 10
@@ -73,6 +74,36 @@ int npph_event_create(void** ev);
 int npph_event_destroy(void* ev);
 int npph_event_record(void* ev, void* stream);
 int npph_event_elapsed_ms(void* start, void* stop, float* ms);
+
+/* ---------------- element-wise ufuncs and reductions ---------------- */
+/* Op codes: the values of GpuBinOp / GpuUnOp / GpuRedOp
+ * (include/numpp/backend/gpu_vtable.hpp:10-12). */
+#define NPPH_EW_ADD 0
+#define NPPH_EW_SUB 1
+#define NPPH_EW_MUL 2
+#define NPPH_EW_DIV 3
+#define NPPH_UN_NEG 0
+#define NPPH_UN_SQRT 1
+#define NPPH_UN_EXP 2
+#define NPPH_RED_SUM 0
+#define NPPH_RED_PROD 1
+/* Replace GpuVTable::elementwise_binary / elementwise_unary / reduce
+ * (gpu_vtable.hpp:19-21; CUDA provider ew_binary / ew_unary / reduce_impl,
+ * src/backend/cuda_backend.cu:270-287; callers ufunc.cpp:525-545, 607-617):
+ * contiguous host buffers of n elements in dtype F32 or F64, synchronous.
+ * Binary and unary results are IEEE-exact (the CPU kernel's, bit for bit);
+ * exp and other dtypes return NPPH_ERR_UNSUPPORTED (the slot returns false and
+ * NumPP's CPU kernel runs, as with the reference's CUDA/OpenCL providers).
+ * reduce writes one element: the sum or product in the array's dtype, within
+ * reduction tolerance of the CPU's sequential loop (ufunc.cpp:450-472). */
+int npph_elementwise_binary(int op, int dtype, int64_t n, const void* a, const void* b, void* out);
+int npph_elementwise_unary(int op, int dtype, int64_t n, const void* a, void* out);
+int npph_reduce(int op, int dtype, int64_t n, const void* a, void* out);
+/* Device-resident forms (device pointers; out of reduce is one device element). */
+int npph_elementwise_binary_dev(int op, int dtype, int64_t n, const void* a, const void* b, void* out,
+                                void* stream);
+int npph_elementwise_unary_dev(int op, int dtype, int64_t n, const void* a, void* out, void* stream);
+int npph_reduce_dev(int op, int dtype, int64_t n, const void* a, void* out, void* stream);
 
 /* ---------------- GEMM ---------------- */
 /* Replaces GpuVTable::gemm (gpu_vtable.hpp:22-26; CUDA provider gemm_impl/
@@ -162,6 +193,22 @@ int npph_inv_batched(int dtype, int64_t count, int64_t n, const void* A, void* A
 int npph_det_batched_dev(int64_t count, int64_t n, double* A, double* det, double* sign, double* logabsdet,
                          int32_t* info, void* stream);
 int npph_inv_batched_dev(int64_t count, int64_t n, double* A, double* Ainv, int32_t* info, void* stream);
+
+/* ---------------- LapackVTable provider (one system, output dtype) ---------------- */
+/* Replace LapackVTable::gesv / inv (include/numpp/backend/lapack_vtable.hpp:
+ * 14-18; reference provider lapack_backend.cpp:15-34), which linalg::solve
+ * tries before its portable kernel (linalg.cpp:372-382). Unlike the portable
+ * kernel they compute in the given dtype: F32 systems are factored and solved
+ * in FP32 (LAPACKE_sgesv's contract), F64 in FP64. Host buffers, row-major;
+ * gesv overwrites A with the packed LU and B (n x nrhs) with X; inv writes
+ * Ainv and leaves A unchanged. An exact-zero pivot returns NPPH_ERR_SINGULAR
+ * (the slot returns false and NumPP's portable kernel raises). */
+int npph_lapack_gesv(int n, int nrhs, int dtype, void* A, void* B);
+int npph_lapack_inv(int n, int dtype, const void* A, void* Ainv);
+/* Address of the numpp::LapackVTable provider struct (layout of
+ * include/numpp_hip/lapack_vtable.hpp: gesv and inv filled, the other slots
+ * null), or NULL without a usable device. */
+const void* npph_lapack_vtable(void);
 
 /* ---------------- mixed precision + FP64 iterative refinement ---------------- */
 /* Options for the refined solver. No reference counterpart (NumPP always

@@ -19,9 +19,14 @@ NPPH_ERR_OOM = -4
 NPPH_ERR_UNSUPPORTED = -5
 NPPH_ERR_BUDGET = -6
 NPPH_ERR_NOT_FOUND = -7
+NPPH_ERR_SINGULAR = -8
 
 DT_F32 = 10
 DT_F64 = 11
+
+EW_ADD, EW_SUB, EW_MUL, EW_DIV = 0, 1, 2, 3
+UN_NEG, UN_SQRT, UN_EXP = 0, 1, 2
+RED_SUM, RED_PROD = 0, 1
 
 PREC_FP64 = 0
 PREC_FP32 = 1
@@ -88,6 +93,12 @@ SIGNATURES = {
     "npph_event_destroy": (_int, [_vp]),
     "npph_event_record": (_int, [_vp, _vp]),
     "npph_event_elapsed_ms": (_int, [_vp, _vp, ctypes.POINTER(_flt)]),
+    "npph_elementwise_binary": (_int, [_int, _int, _i64, _vp, _vp, _vp]),
+    "npph_elementwise_unary": (_int, [_int, _int, _i64, _vp, _vp]),
+    "npph_reduce": (_int, [_int, _int, _i64, _vp, _vp]),
+    "npph_elementwise_binary_dev": (_int, [_int, _int, _i64, _vp, _vp, _vp, _vp]),
+    "npph_elementwise_unary_dev": (_int, [_int, _int, _i64, _vp, _vp, _vp]),
+    "npph_reduce_dev": (_int, [_int, _int, _i64, _vp, _vp, _vp]),
     "npph_gemm": (_int, [_int, _i64, _i64, _i64, _vp, _vp, _vp]),
     "npph_dgemm_dev": (_int, [_i64, _i64, _i64, _dbl, _vp, _i64, _i64, _vp, _i64, _i64, _dbl, _vp, _i64, _i64, _i64, _vp]),
     "npph_sgemm_dev": (_int, [_i64, _i64, _i64, _flt, _vp, _i64, _i64, _vp, _i64, _i64, _flt, _vp, _i64, _i64, _i64, _vp]),
@@ -116,6 +127,9 @@ SIGNATURES = {
     "npph_prof_reset": (_int, []),
     "npph_prof_get": (_int, [_int, ctypes.POINTER(_dbl), ctypes.POINTER(_dbl), ctypes.POINTER(_dbl), ctypes.POINTER(_i64)]),
     "npph_gpu_vtable": (_vp, []),
+    "npph_lapack_vtable": (_vp, []),
+    "npph_lapack_gesv": (_int, [_int, _int, _int, _vp, _vp]),
+    "npph_lapack_inv": (_int, [_int, _int, _vp, _vp]),
 }
 
 _lib = None

@@ -4,16 +4,29 @@
 // slot functions and accessor): every slot forwards to the C ABI in
 // numpp_hip.h and returns false on any failure so NumPP's dispatcher falls back
 // to its CPU kernel (gpu_vtable.hpp:14-16, backend.cpp:148-152). Slots never
-// throw. Unimplemented reference slots (elementwise, reduce, ScyPP) are null,
-// which the reference dispatcher already treats as "no device path".
+// throw. The ScyPP slots (csr_spmv, pairwise_sqdist, separable_corr1d) are
+// null, which the reference dispatcher already treats as "no device path".
 #include <cstring>
 #include <vector>
 
 #include "numpp_hip.h"
 #include "numpp_hip/gpu_vtable.hpp"
+#include "numpp_hip/lapack_vtable.hpp"
 
 namespace numpp {
 namespace {
+
+bool ew_binary_slot(int op, DTypeId dt, int64_t n, const void* a, const void* b, void* out) {
+  return npph_elementwise_binary(op, static_cast<int>(dt), n, a, b, out) == NPPH_OK;
+}
+
+bool ew_unary_slot(int op, DTypeId dt, int64_t n, const void* a, void* out) {
+  return npph_elementwise_unary(op, static_cast<int>(dt), n, a, out) == NPPH_OK;
+}
+
+bool reduce_slot(int op, DTypeId dt, int64_t n, const void* a, void* out) {
+  return npph_reduce(op, static_cast<int>(dt), n, a, out) == NPPH_OK;
+}
 
 bool gemm_slot(DTypeId dt, int64_t m, int64_t n, int64_t k, const void* A, const void* B, void* C) {
   return npph_gemm(static_cast<int>(dt), m, n, k, A, B, C) == NPPH_OK;
@@ -70,11 +83,25 @@ bool inv_slot(DTypeId dt, int64_t count, int64_t n, const void* A, void* Ainv, i
 }
 
 const GpuVTable g_vtable{
-    "hip(gfx950)", nullptr, nullptr, nullptr, &gemm_slot, nullptr, nullptr, nullptr,
+    "hip(gfx950)", &ew_binary_slot, &ew_unary_slot, &reduce_slot, &gemm_slot, nullptr, nullptr, nullptr,
     &gesv_batched_slot, &gesv_refined_slot, &det_slot, &slogdet_slot, &inv_slot,
 };
 
+bool lapack_gesv_slot(int n, int nrhs, DTypeId dt, void* A, void* B) {
+  return npph_lapack_gesv(n, nrhs, static_cast<int>(dt), A, B) == NPPH_OK;
+}
+
+bool lapack_inv_slot(int n, DTypeId dt, const void* A, void* Ainv) {
+  return npph_lapack_inv(n, static_cast<int>(dt), A, Ainv) == NPPH_OK;
+}
+
+const LapackVTable g_lapack{&lapack_gesv_slot, &lapack_inv_slot, nullptr, nullptr, nullptr, nullptr};
+
 }  // namespace
+
+// The LAPACK provider of a NumPP build that selects this backend for
+// lapack_vtable() (lapack_vtable.hpp:30); null without a usable device.
+const LapackVTable* lapack_vtable() { return npph_available() ? &g_lapack : nullptr; }
 
 // Probed once per process (npph_init uses std::call_once); null without a
 // usable gfx950 device, the reference's "no device" contract (gpu_vtable.hpp:56).
@@ -83,3 +110,4 @@ const GpuVTable* gpu_vtable() { return npph_available() ? &g_vtable : nullptr; }
 }  // namespace numpp
 
 extern "C" const void* npph_gpu_vtable(void) { return numpp::gpu_vtable(); }
+extern "C" const void* npph_lapack_vtable(void) { return numpp::lapack_vtable(); }

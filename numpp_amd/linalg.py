@@ -31,6 +31,22 @@ class LinAlgError(ValueError):
     """Mirror of numpp::linalg_error (include/numpp/core/error.hpp:43-46)."""
 
 
+_DTID = {np.dtype(np.float32): _lib.DT_F32, np.dtype(np.float64): _lib.DT_F64}
+
+
+def lapack_vtable():
+    """The LapackVTable a NumPP build would link when the HIP provider is chosen
+    for lapack_vtable() (the NUMPP_LAPACK_BACKEND=hip analogue of
+    NUMPP_WITH_LAPACK), else None: the portable kernel runs."""
+    import os
+    if os.environ.get("NUMPP_LAPACK_BACKEND") != "hip":
+        return None
+    try:
+        return vtable.get_lapack()
+    except _lib.BackendUnavailable:
+        return None
+
+
 def _out_dtype(a, b):
     rt = np.result_type(a.dtype, b.dtype)
     return np.float32 if rt == np.float32 else np.float64
@@ -67,6 +83,15 @@ def solve(a, b, forced: Backend = Backend.Auto, cpu_kernel=None):
                 if np.any(info != 0):
                     raise LinAlgError("Singular matrix")
                 return B.reshape(exp_b).astype(out_dt, copy=False)
+    # A linked LAPACK provider (linalg.cpp:372-382): 2-D systems only, computed
+    # in the output dtype; a false return (e.g. singular) falls through.
+    lvt = lapack_vtable() if a.ndim == 2 else None
+    if lvt is not None and lvt.gesv:
+        A = np.ascontiguousarray(a, dtype=out_dt).copy()
+        B = np.ascontiguousarray(b, dtype=out_dt).reshape(n, nrhs).copy()
+        if lvt.gesv(n, nrhs, _DTID[np.dtype(out_dt)], A.ctypes.data, B.ctypes.data):
+            _set_last(Backend.Cpu)
+            return B.reshape(b.shape)
     _set_last(Backend.Cpu)
     if cpu_kernel is None:
         raise CpuPathSelected("solve dispatched to NumPP's CPU kernel")
@@ -159,6 +184,44 @@ def inv(a, forced: Backend = Backend.Auto, cpu_kernel=None):
                     raise LinAlgError("Singular matrix")
                 return out.reshape(a.shape).astype(out_dt, copy=False)
     return _cpu("inv", cpu_kernel, a)
+
+
+def tensorsolve(a, b, axes=None, forced: Backend = Backend.Auto, cpu_kernel=None):
+    """numpp::linalg::tensorsolve (tensor_solve.cpp:10-32): move the named axes
+    of a to the end, square the trailing axes against b's shape, and solve the
+    (prod, prod) system through solve() (which takes the GPU hook)."""
+    a = np.asarray(a)
+    b = np.asarray(b)
+    if axes:
+        allaxes = list(range(a.ndim))
+        for k in axes:
+            allaxes.remove(k)
+            allaxes.append(k)
+        a = a.transpose(allaxes)
+    if a.ndim < b.ndim:
+        raise LinAlgError("tensorsolve: a.ndim must be >= b.ndim")
+    oldshape = a.shape[b.ndim:] if a.ndim > b.ndim else ()
+    prod = int(np.prod(oldshape, dtype=np.int64))
+    if a.size != prod * prod:
+        raise LinAlgError("tensorsolve: prod(a.shape[b.ndim:]) must equal prod(a.shape[:b.ndim])")
+    A = np.ascontiguousarray(a).reshape(prod, prod)
+    B = np.ascontiguousarray(b).reshape(prod)
+    kern = None if cpu_kernel is None else (lambda x, y: cpu_kernel(x, y))
+    return solve(A, B, forced, kern).reshape(oldshape)
+
+
+def tensorinv(a, ind=2, forced: Backend = Backend.Auto, cpu_kernel=None):
+    """numpp::linalg::tensorinv (tensor_solve.cpp:34-49): inverse of a reshaped
+    to (prod(shape[ind:]), prod(shape[:ind])) through inv(), reshaped to
+    shape[ind:] + shape[:ind]."""
+    a = np.asarray(a)
+    if ind <= 0 or ind > a.ndim:
+        raise ValueError("Invalid ind argument")
+    sh = a.shape
+    prod = int(np.prod(sh[ind:], dtype=np.int64))
+    lead = int(np.prod(sh[:ind], dtype=np.int64))
+    A = np.ascontiguousarray(a).reshape(prod, lead)
+    return inv(A, forced, cpu_kernel).reshape(sh[ind:] + sh[:ind])
 
 
 # ---------------------------------------------------------------------------

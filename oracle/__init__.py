@@ -61,6 +61,14 @@ def _load():
     lib.orc_matmul_numpp.restype = None
     lib.orc_nd_copy.argtypes = [_dp, ctypes.c_int64, ctypes.c_int64]
     lib.orc_nd_copy.restype = ctypes.c_void_p
+    lib.orc_reduce_f64.argtypes = [ctypes.c_int, ctypes.c_int64, _dp]
+    lib.orc_reduce_f64.restype = ctypes.c_double
+    lib.orc_reduce_f32.argtypes = [ctypes.c_int, ctypes.c_int64, _fp]
+    lib.orc_reduce_f32.restype = ctypes.c_float
+    lib.orc_ew_f64.argtypes = [ctypes.c_int, ctypes.c_int64, _dp, _dp, _dp]
+    lib.orc_ew_f64.restype = None
+    lib.orc_ew_f32.argtypes = [ctypes.c_int, ctypes.c_int64, _fp, _fp, _fp]
+    lib.orc_ew_f32.restype = None
     lib.orc_det.argtypes = [_dp, ctypes.c_int]
     lib.orc_det.restype = ctypes.c_double
     lib.orc_slogdet.argtypes = [_dp, ctypes.c_int, _dp, _dp]
@@ -156,6 +164,33 @@ def matmul_numpp(a, b):
     c = np.empty((a.shape[0], b.shape[1]))
     _load().orc_matmul_numpp(a.shape[0], b.shape[1], a.shape[1], _ptr(a), _ptr(b), _ptr(c))
     return c
+
+
+EW_OPS = {"add": 0, "subtract": 1, "multiply": 2, "divide": 3, "negative": 10, "sqrt": 11}
+
+
+def elementwise(name, a, b=None):
+    """NumPP's CPU element-wise arithmetic in the array's dtype (float32/64)."""
+    a = np.ascontiguousarray(a)
+    f32 = a.dtype == np.float32
+    a = _c(a, np.float32 if f32 else np.float64)
+    b = a if b is None else _c(b, a.dtype)
+    o = np.empty_like(a)
+    if f32:
+        _load().orc_ew_f32(EW_OPS[name], a.size, _ptr(a, _fp), _ptr(b, _fp), _ptr(o, _fp))
+    else:
+        _load().orc_ew_f64(EW_OPS[name], a.size, _ptr(a), _ptr(b), _ptr(o))
+    return o
+
+
+def reduce(name, a):
+    """ufunc.cpp:450-472: sequential sum / prod in the array's dtype."""
+    op = {"sum": 0, "prod": 1}[name]
+    a = np.ascontiguousarray(a).ravel()
+    if a.dtype == np.float32:
+        return np.float32(_load().orc_reduce_f32(op, a.size, _ptr(a, _fp)))
+    a = _c(a)
+    return _load().orc_reduce_f64(op, a.size, _ptr(a))
 
 
 def det(a):

@@ -23,6 +23,10 @@
  *   orc_batched_solve_numpp <- batched.cpp:90-100 with the ndarray copies each
  *                         matrix goes through (copy(), to_vec, from_vec, stack)
  *   orc_matmul_numpp   <- backend.cpp:55-84 with its two astype copies
+ *   orc_reduce_f64/f32 <- src/umath/ufunc.cpp:450-472 (reduce_row_numeric, sum
+ *                         and prod: one sequential accumulator in the dtype)
+ *   orc_ew_f64/f32     <- the portable element-wise kernels' arithmetic
+ *                         (add, subtract, multiply, divide, negative, sqrt)
  * Helpers that NumPP does not have (pinned by mathematics, not by NumPP):
  *   orc_dot2_entry     compensated dot product (Ogita-Rump-Oishi Dot2) of one
  *                      GEMM entry, used to bound GEMM error at sizes where the
@@ -277,6 +281,47 @@ void orc_matmul_numpp(int64_t m, int64_t n, int64_t k, const double* A, const do
   double* bc = orc_nd_copy(B, k, n);
   orc_matmul(m, n, k, ac, bc, C);
   free(ac); free(bc);
+}
+
+/* ufunc.cpp:450-472 for Sum (op 0) and Prod (op 1): acc starts at 0 or 1 and
+ * takes every element in order. */
+double orc_reduce_f64(int op, int64_t n, const double* p) {
+  double acc = op == 1 ? 1.0 : 0.0;
+  for (int64_t i = 0; i < n; ++i) acc = op == 1 ? acc * p[i] : acc + p[i];
+  return acc;
+}
+float orc_reduce_f32(int op, int64_t n, const float* p) {
+  float acc = op == 1 ? 1.0f : 0.0f;
+  for (int64_t i = 0; i < n; ++i) acc = op == 1 ? acc * p[i] : acc + p[i];
+  return acc;
+}
+
+/* Element-wise arithmetic as NumPP's CPU kernels compute it (IEEE operations
+ * in the dtype): binary ops 0..3 = add, subtract, multiply, divide; unary ops
+ * 10 = negative, 11 = sqrt (b unused). */
+void orc_ew_f64(int op, int64_t n, const double* a, const double* b, double* o) {
+  for (int64_t i = 0; i < n; ++i) {
+    switch (op) {
+      case 0: o[i] = a[i] + b[i]; break;
+      case 1: o[i] = a[i] - b[i]; break;
+      case 2: o[i] = a[i] * b[i]; break;
+      case 3: o[i] = a[i] / b[i]; break;
+      case 10: o[i] = -a[i]; break;
+      default: o[i] = sqrt(a[i]); break;
+    }
+  }
+}
+void orc_ew_f32(int op, int64_t n, const float* a, const float* b, float* o) {
+  for (int64_t i = 0; i < n; ++i) {
+    switch (op) {
+      case 0: o[i] = a[i] + b[i]; break;
+      case 1: o[i] = a[i] - b[i]; break;
+      case 2: o[i] = a[i] * b[i]; break;
+      case 3: o[i] = a[i] / b[i]; break;
+      case 10: o[i] = -a[i]; break;
+      default: o[i] = sqrtf(a[i]); break;
+    }
+  }
 }
 
 /* backend.cpp:55-84 for float32 (compute dtype stays float32). */

@@ -12,6 +12,12 @@ Fixtures (all .npz / .npy, loadable with allow_pickle=False):
   config1_x.npy       np.linalg.solve on the 256 x 64 x 64 stacked workload
                       (A = U[-1,1) + 64 I, b ~ N(0,1), default_rng(0)).
   matmul_cases.npz    seeded float64 / float32 GEMMs with numpy results.
+  sgesv_cases.npz     seeded float32 systems with scipy.linalg.lu_factor /
+                      lu_solve in float32 (LAPACK sgetrf / sgetrs): ipiv,
+                      packed LU and X, for the float32 LapackVTable contract.
+  tensor_cases.npz    np.linalg.tensorsolve / tensorinv on the reference's
+                      tridiagonal 2x2x2x2 literal (test_tensor_solve.cpp:13-39)
+                      and on a seeded 3x4x12 case.
   det_cases.npz       seeded general matrices (rows shuffled so pivoting
                       swaps) with np.linalg.det, np.linalg.slogdet and
                       np.linalg.inv (LAPACK dgetrf / dgesv).
@@ -101,6 +107,36 @@ def det_cases():
     np.savez(os.path.join(HERE, "det_cases.npz"), **out)
 
 
+def sgesv_cases():
+    rng = np.random.default_rng(2026)
+    sizes = [3, 16, 40, 64, 65, 100, 200]
+    out = {}
+    for n in sizes:
+        a = rng.uniform(-1.0, 1.0, (n, n)) + np.diag(rng.uniform(2.0, 3.0, n) * np.sign(rng.uniform(-1, 1, n)))
+        a = a[rng.permutation(n)].astype(np.float32)
+        b = rng.standard_normal((n, 3)).astype(np.float32)
+        lu, ipiv = sla.lu_factor(a)
+        assert lu.dtype == np.float32
+        out[f"A_{n}"] = a
+        out[f"B_{n}"] = b
+        out[f"LU_{n}"] = lu
+        out[f"ipiv_{n}"] = ipiv.astype(np.int32)
+        out[f"X_{n}"] = sla.lu_solve((lu, ipiv), b)
+    out["sizes"] = np.array(sizes, np.int32)
+    np.savez(os.path.join(HERE, "sgesv_cases.npz"), **out)
+
+
+def tensor_cases():
+    m = np.array([4, 1, 0, 0, 1, 4, 1, 0, 0, 1, 4, 1, 0, 0, 1, 4.0]).reshape(2, 2, 2, 2)
+    b = np.array([1, 2, 3, 4.0]).reshape(2, 2)
+    rng = np.random.default_rng(2027)
+    a2 = rng.standard_normal((3, 4, 12)) + np.eye(12).reshape(3, 4, 12) * 4.0
+    b2 = rng.standard_normal((3, 4))
+    np.savez(os.path.join(HERE, "tensor_cases.npz"), M=m, b=b, x=np.linalg.tensorsolve(m, b),
+             Minv=np.linalg.tensorinv(m, 2), A2=a2, b2=b2, x2=np.linalg.tensorsolve(a2, b2),
+             A2inv=np.linalg.tensorinv(a2, 2))
+
+
 def reference_golden():
     src = "/root/reference/tests/golden/g15653376641833044941.npy"
     dst = os.path.join(HERE, "numpp_g15653376641833044941.npy")
@@ -114,5 +150,7 @@ if __name__ == "__main__":
     config1()
     matmul_cases()
     det_cases()
+    sgesv_cases()
+    tensor_cases()
     reference_golden()
     print("goldens written to", HERE)

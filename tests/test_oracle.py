@@ -318,3 +318,42 @@ def test_numpp_equivalent_timing_paths_compute_the_same():
     rng = np.random.default_rng(13)
     A, B = rng.standard_normal((30, 20)), rng.standard_normal((20, 10))
     np.testing.assert_array_equal(oracle.matmul_numpp(A, B), oracle.matmul(A, B))
+
+
+# ---------------- element-wise ufuncs and reductions: ufunc.cpp:450-472 ----------------
+
+def _specials(dt):
+    f = np.finfo(dt)
+    return np.array([0.0, -0.0, 1.0, -1.0, np.inf, -np.inf, np.nan, f.tiny, -f.tiny, f.smallest_subnormal,
+                     f.max, -f.max, 3.0, 1e-3, 7.5], dtype=dt)
+
+
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+def test_elementwise_matches_numpy_ieee(dt):
+    rng = np.random.default_rng(21)
+    s = _specials(dt)
+    a = np.concatenate([np.repeat(s, s.size), rng.standard_normal(500).astype(dt)])
+    b = np.concatenate([np.tile(s, s.size), rng.standard_normal(500).astype(dt)])
+    with np.errstate(all="ignore"):
+        want = {"add": a + b, "subtract": a - b, "multiply": a * b, "divide": a / b, "negative": -a,
+                "sqrt": np.sqrt(a)}
+    for name, w in want.items():
+        got = oracle.elementwise(name, a, None if name in ("negative", "sqrt") else b)
+        assert got.dtype == dt
+        np.testing.assert_array_equal(np.isnan(got), np.isnan(w))
+        ok = ~np.isnan(w)
+        assert np.array_equal(got[ok].view(np.uint32 if dt == np.float32 else np.uint64),
+                              w[ok].view(np.uint32 if dt == np.float32 else np.uint64)), name
+
+
+def test_reduce_sequential_order():
+    # exact on small integers; the sequential order is visible where rounding
+    # depends on it: ((1e16 + 1) + 1) - 1e16 = 0 in order, 2 exactly
+    x = np.arange(1.0, 1001.0)
+    assert oracle.reduce("sum", x) == 500500.0
+    assert oracle.reduce("sum", np.array([1e16, 1.0, 1.0, -1e16])) == 0.0
+    assert oracle.reduce("prod", np.array([2.0, 0.5, 4.0, 3.0])) == 12.0
+    assert oracle.reduce("sum", np.array([], np.float64)) == 0.0 and oracle.reduce("prod", np.array([])) == 1.0
+    f = np.full(2 ** 25, 1.0, np.float32)
+    # float32 accumulation stalls at 2^24 (1 + 2^24 rounds to 2^24): the dtype is the accumulator's
+    assert oracle.reduce("sum", f) == np.float32(2 ** 24)
