@@ -58,6 +58,8 @@ int npph_available(void);
 const char* npph_last_error(void);
 const char* npph_device_name(void);
 int npph_cu_count(void);
+/* HIP device ordinal the library runs on (the DLPack device_id of its arrays). */
+int npph_device_id(void);
 
 /* Device memory / stream plumbing for callers that keep data resident. */
 int npph_malloc(void** ptr, uint64_t bytes);

@@ -79,6 +79,7 @@ SIGNATURES = {
     "npph_last_error": (ctypes.c_char_p, []),
     "npph_device_name": (ctypes.c_char_p, []),
     "npph_cu_count": (_int, []),
+    "npph_device_id": (_int, []),
     "npph_malloc": (_int, [ctypes.POINTER(_vp), ctypes.c_uint64]),
     "npph_free": (_int, [_vp]),
     "npph_memcpy_htod": (_int, [_vp, _vp, ctypes.c_uint64, _vp]),

@@ -154,50 +154,91 @@ __device__ __forceinline__ V piv_key(V a, int pos, int k) {
   return a != a ? (pos == k ? V(INFINITY) : V(-1)) : fabs(a);
 }
 
-// One exchange step of the argmax: each lane compares with the lane that the
-// DPP pattern CTRL names (all patterns used stay inside a 16-lane row).
+// Value of lane (this lane's position in a DPP pattern CTRL); all patterns used
+// stay inside a 16-lane row.
 template <int CTRL>
-__device__ __forceinline__ void argmax_dpp(double& v, int& pos) {
-  const int2 u = __builtin_bit_cast(int2, v);
-  int2 o;
-  o.x = __builtin_amdgcn_mov_dpp(u.x, CTRL, 0xF, 0xF, false);
-  o.y = __builtin_amdgcn_mov_dpp(u.y, CTRL, 0xF, 0xF, false);
-  const int op = __builtin_amdgcn_mov_dpp(pos, CTRL, 0xF, 0xF, false);
-  const double ov = __builtin_bit_cast(double, o);
-  if (piv_better(ov, op, v, pos)) { v = ov; pos = op; }
+__device__ __forceinline__ float dpp_f(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), CTRL, 0xF, 0xF, false));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_f(double v) {
+  int2 u = __builtin_bit_cast(int2, v);
+  u.x = __builtin_amdgcn_mov_dpp(u.x, CTRL, 0xF, 0xF, false);
+  u.y = __builtin_amdgcn_mov_dpp(u.y, CTRL, 0xF, 0xF, false);
+  return __builtin_bit_cast(double, u);
+}
+template <int CTRL>
+__device__ __forceinline__ int dpp_i(int v) {
+  return __builtin_amdgcn_mov_dpp(v, CTRL, 0xF, 0xF, false);
 }
 
-template <int CTRL>
-__device__ __forceinline__ void argmax_dpp(float& v, int& pos) {
-  const int o = __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), CTRL, 0xF, 0xF, false);
-  const int op = __builtin_amdgcn_mov_dpp(pos, CTRL, 0xF, 0xF, false);
-  const float ov = __builtin_bit_cast(float, o);
-  if (piv_better(ov, op, v, pos)) { v = ov; pos = op; }
-}
-
-// Wave-wide argmax of (v, pos); every lane receives the winner. Within each
-// 16-lane row: DPP swaps with the neighbouring lane, the neighbouring pair,
+// Wave-wide maximum of a value that is never NaN, and minimum of an int: within
+// each 16-lane row, DPP swaps with the neighbouring lane, the neighbouring pair,
 // the mirrored half-row and the mirrored row (quad_perm [1,0,3,2], quad_perm
-// [2,3,0,1], row_half_mirror, row_mirror) leave the row's winner in all of
-// its lanes; the four row winners are then read with v_readlane. The order is
-// total (largest value, then smallest position), so the result does not
-// depend on the reduction tree.
+// [2,3,0,1], row_half_mirror, row_mirror) leave the row's result in all of its
+// lanes; the four row results are then combined with v_readlane. One compare
+// per stage.
+template <class V>
+__device__ __forceinline__ V wave_max(V v) {
+  V o;
+  o = dpp_f<0xB1>(v); v = o > v ? o : v;
+  o = dpp_f<0x4E>(v); v = o > v ? o : v;
+  o = dpp_f<0x141>(v); v = o > v ? o : v;
+  o = dpp_f<0x140>(v); v = o > v ? o : v;
+  const V r0 = readlane<V>(v, 0), r1 = readlane<V>(v, 16), r2 = readlane<V>(v, 32), r3 = readlane<V>(v, 48);
+  const V a = r1 > r0 ? r1 : r0, b = r3 > r2 ? r3 : r2;
+  return b > a ? b : a;
+}
+__device__ __forceinline__ int wave_min_i(int v) {
+  v = min(v, dpp_i<0xB1>(v));
+  v = min(v, dpp_i<0x4E>(v));
+  v = min(v, dpp_i<0x141>(v));
+  v = min(v, dpp_i<0x140>(v));
+  return min(min(__builtin_amdgcn_readlane(v, 0), __builtin_amdgcn_readlane(v, 16)),
+             min(__builtin_amdgcn_readlane(v, 32), __builtin_amdgcn_readlane(v, 48)));
+}
+
+// The same argmax over lanes 0..7 only (the other lanes' values are ignored):
+// three DPP stages (quad_perm [1,0,3,2], quad_perm [2,3,0,1], row_half_mirror)
+// leave the result of lanes 0..7 in lane 0.
+template <class V>
+__device__ __forceinline__ void lanes8_argmax(V& v, int& pos) {
+  const int lane = threadIdx.x & 63;
+  V x = v, o;
+  o = dpp_f<0xB1>(x); x = o > x ? o : x;
+  o = dpp_f<0x4E>(x); x = o > x ? o : x;
+  o = dpp_f<0x141>(x); x = o > x ? o : x;
+  const V wm = readlane<V>(x, 0);
+  const uint64_t m = __ballot(lane < 8 && v == wm);
+  int wp;
+  if (__popcll(m) == 1) {
+    wp = __builtin_amdgcn_readlane(pos, (int)__builtin_ctzll(m));
+  } else {
+    int q = (lane < 8 && v == wm) ? pos : 0x7fffffff;
+    q = min(q, dpp_i<0xB1>(q));
+    q = min(q, dpp_i<0x4E>(q));
+    q = min(q, dpp_i<0x141>(q));
+    wp = __builtin_amdgcn_readlane(q, 0);
+  }
+  v = wm;
+  pos = wp;
+}
+
+// Wave-wide argmax of (v, pos) with NumPP's pivot rule (linalg.cpp:55-56): the
+// largest v, and among equal values the smallest position. Every lane receives
+// the winner. v is never NaN (pivot keys map NaN to +inf or -1, piv_key), so
+// the maximum is exact and order-independent; the lanes holding it are found
+// with a ballot, and only when several hold it (ties) is their minimum position
+// reduced as well.
 template <class V>
 __device__ __forceinline__ void wave_argmax(V& v, int& pos) {
-  argmax_dpp<0xB1>(v, pos);
-  argmax_dpp<0x4E>(v, pos);
-  argmax_dpp<0x141>(v, pos);
-  argmax_dpp<0x140>(v, pos);
-  V bv = readlane<V>(v, 0);
-  int bp = __builtin_amdgcn_readlane(pos, 0);
-#pragma unroll
-  for (int r = 1; r < 4; ++r) {
-    const V ov = readlane<V>(v, 16 * r);
-    const int op = __builtin_amdgcn_readlane(pos, 16 * r);
-    if (piv_better(ov, op, bv, bp)) { bv = ov; bp = op; }
-  }
-  v = bv;
-  pos = bp;
+  const V wm = wave_max(v);
+  const uint64_t m = __ballot(v == wm);
+  int wp;
+  if (__popcll(m) == 1) wp = __builtin_amdgcn_readlane(pos, (int)__builtin_ctzll(m));
+  else wp = wave_min_i(v == wm ? pos : 0x7fffffff);
+  v = wm;
+  pos = wp;
 }
 
 }  // namespace npph

@@ -65,12 +65,16 @@ __global__ __launch_bounds__(NTH) void leaf_kernel(LeafArgs<T> a) {
   __shared__ T Ublk[LEFT][IB];
   __shared__ T red_v[2][NWL];  // candidate magnitudes (exact in T)
   __shared__ int red_p[2][NWL];
-  __shared__ T red_row[2][NWL][IB];
+  __shared__ __attribute__((aligned(16))) T red_row[2][NWL][IB];
   __shared__ int mv_old[2 * IB], mv_new[2 * IB];
   __shared__ int mv_cnt;
   __shared__ T stage[2 * IB][LEFT];
   __shared__ int stage_perm[2 * IB], stage_pid[2 * IB];
   __shared__ int sing_s;
+  // pivot rows of the leaf's steps, written to HBM once after the last block:
+  // a global store inside the column loop would make every column barrier
+  // wait for it (the barrier's release fence)
+  __shared__ int piv_s[LEAF];
   extern __shared__ int pid[];  // m entries: leaf-start row of the row now at each position
 
   for (int i = t; i < m; i += NTH) pid[i] = i;
@@ -186,31 +190,34 @@ __global__ __launch_bounds__(NTH) void leaf_kernel(LeafArgs<T> a) {
       wave_argmax(wv, wp);
       if (bs >= 0 && wp == bp) {
         // this lane owns the wave's candidate row: offer it with its values
+        // (from a 16-byte boundary, so the stores are 16-byte vectors; the
+        // entries left of c are never read)
+        constexpr int C0 = c & ~(16 / (int)sizeof(T) - 1);
         red_v[buf][wave] = wv;
         red_p[buf][wave] = wp;
 #pragma unroll
         for (int s = 0; s < R; ++s)
           if (s == bs) {
 #pragma unroll
-            for (int j = c; j < IB; ++j) red_row[buf][wave][j] = v[s][j];
+            for (int j = C0; j < IB; ++j) red_row[buf][wave][j] = v[s][j];
           }
       } else if (lane == 0 && wv < T(0)) {
         red_v[buf][wave] = T(-1);
         red_p[buf][wave] = 1 << 28;
       }
       __syncthreads();
-      int wbest = 0;
-      bv = red_v[buf][0];
-      bp = red_p[buf][0];
-#pragma unroll
-      for (int q = 1; q < NWL; ++q) {
-        const T ov = red_v[buf][q];
-        const int op = red_p[buf][q];
-        if (piv_better(ov, op, bv, bp)) { bv = ov; bp = op; wbest = q; }
-      }
+      // cross-wave pick: lane q < NWL reads wave q's candidate, then the same
+      // wave argmax (the row at position col is a candidate of some wave, so
+      // the winner is a real row and its wave is unique)
+      const int cp = lane < NWL ? red_p[buf][lane < NWL ? lane : 0] : (1 << 29);
+      bv = lane < NWL ? red_v[buf][lane < NWL ? lane : 0] : T(-2);
+      bp = cp;
+      static_assert(NWL <= 8, "the cross-wave pick reduces over lanes 0..7");
+      lanes8_argmax(bv, bp);
+      const int wbest = (int)__builtin_ctzll(__ballot(lane < NWL && cp == bp));
       const int p = bp;
       if (bv == T(0) && t == 0 && sing_s == 0) sing_s = k0 + col + 1;
-      if (t == 0) ipiv[k0 + col] = k0 + p;
+      if (t == 0) piv_s[col] = k0 + p;
       const T piv = red_row[buf][wbest][c];
       // FP32 factors (refined afterwards) scale by one reciprocal per column;
       // FP64 factors divide, as linalg.cpp:65
@@ -220,13 +227,19 @@ __global__ __launch_bounds__(NTH) void leaf_kernel(LeafArgs<T> a) {
         if (pos[s] == p) pos[s] = col;
         else if (pos[s] == col) pos[s] = p;
       }
+      // the pivot row's entries right of c, read once for all R rows (from a
+      // 16-byte boundary: vector loads)
+      constexpr int C1 = (c + 1) & ~(16 / (int)sizeof(T) - 1);
+      T urow[IB - C1 > 0 ? IB - C1 : 1];
+#pragma unroll
+      for (int j = C1; j < IB; ++j) urow[j - C1] = red_row[buf][wbest][j];
 #pragma unroll
       for (int s = 0; s < R; ++s) {
         if (pos[s] > col && pos[s] < m) {
           const T l = sizeof(T) == 4 ? v[s][c] * rpiv : (piv != T(0) ? v[s][c] / piv : T(0));
           v[s][c] = l;
 #pragma unroll
-          for (int j = c + 1; j < IB; ++j) v[s][j] = v[s][j] - l * red_row[buf][wbest][j];
+          for (int j = c + 1; j < IB; ++j) v[s][j] = v[s][j] - l * urow[j - C1];
         }
       }
     };
@@ -323,6 +336,7 @@ __global__ __launch_bounds__(NTH) void leaf_kernel(LeafArgs<T> a) {
 
   // Net row moves of this leaf for the columns outside it.
   if (t == 0) { mv_cnt = 0; a.info[mat] = sing_s; }
+  for (int c = t; c < w; c += NTH) ipiv[k0 + c] = piv_s[c];
   __syncthreads();
   int32_t* mv = a.mv + mat * a.mv_stride;
   for (int i = t; i < m; i += NTH) {

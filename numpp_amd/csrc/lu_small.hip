@@ -49,6 +49,8 @@ template <int N>
 struct SmallShared {
   double l[2][64];       // multipliers of the step (by lane), even / odd steps
   int p[2], wl[2];       // pivot position and pivot lane of the step
+  int piv[N];            // pivot rows, written to HBM after the loop (a global
+                         // store inside it would make each step's barrier wait)
   int sing;              // first step with an exact-zero pivot, or -1
   double L[N][N + 1];    // strictly lower factor by logical row (zeros elsewhere)
   double U[N][N + 1];    // strictly upper factor by logical row, column k / u_kk
@@ -58,8 +60,7 @@ struct SmallShared {
 
 // Factorisation, as seen by wave W (all arguments uniform across the wave).
 template <int N, int W>
-__device__ __forceinline__ void factor(SmallShared<N>& sh, double (&a)[Small<N>::CPW], int& pos, int n, int lane,
-                                       int32_t* ipiv) {
+__device__ __forceinline__ void factor(SmallShared<N>& sh, double (&a)[Small<N>::CPW], int& pos, int n, int lane) {
   constexpr int CPW = Small<N>::CPW;
   bool stop = false;
   auto step = [&]<int K>() __attribute__((always_inline)) {
@@ -85,7 +86,7 @@ __device__ __forceinline__ void factor(SmallShared<N>& sh, double (&a)[Small<N>:
       if (lane == 0) {
         sh.p[BUF] = p;
         sh.wl[BUF] = wl;
-        if (ipiv) ipiv[K] = p;
+        sh.piv[K] = p;
         if (v == 0.0) sh.sing = K;
       }
     }
@@ -130,9 +131,15 @@ __device__ __forceinline__ void wave_body(const UniformBatch& u, SmallShared<N>&
     }
   }
   int pos = lane < N ? lane : (1 << 20);
-  factor<N, W>(sh, a, pos, n, lane, u.ipiv ? u.ipiv + mat * n : nullptr);
+  factor<N, W>(sh, a, pos, n, lane);
   const bool singular = sh.sing >= 0;
   if (W == 0 && lane == 0 && u.info) u.info[mat] = singular ? sh.sing + 1 : 0;
+  if (W == 0 && u.ipiv) {
+    // steps after an exact-zero pivot were not run: their pivot rows stay
+    // unspecified, as the reference stops there
+    const int ks = singular ? sh.sing + 1 : n;
+    if (lane < ks) u.ipiv[mat * n + lane] = sh.piv[lane];
+  }
   if (u.LU && lane < n && pos < n) {
     double* o = u.LU + mat * u.strideA + (int64_t)pos * n;
 #pragma unroll

@@ -230,6 +230,10 @@ int npph_cu_count(void) {
   ensure_init();
   return ctx().cus;
 }
+int npph_device_id(void) {
+  ensure_init();
+  return ctx().device;
+}
 
 int npph_malloc(void** ptr, uint64_t bytes) {
   NPPH_TRY(ensure_init());

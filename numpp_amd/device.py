@@ -79,13 +79,8 @@ def _capsule_dtor(cap):
 
 
 def current_device() -> int:
-    dev = ctypes.c_int(0)
-    try:
-        hip = ctypes.CDLL("libamdhip64.so")
-        hip.hipGetDevice(ctypes.byref(dev))
-    except OSError:
-        pass
-    return dev.value
+    """The device the library runs on (npph_device_id)."""
+    return int(_lib.load().npph_device_id())
 
 
 class Stream:
