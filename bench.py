@@ -578,13 +578,14 @@ def bench_serve(args):
     the same --requests requests (cycling over --distinct generated systems)
     replayed per rate (serve_trace). Budget 32e9
     bytes (32 GB = 29.8 GiB), 4 workers, batches of up to 8 same-bucket
-    requests. Mixed (FP32 + refinement) and plain FP64. The roofline is the
-    dominant kernel class of one extra replay at the lowest rate with per-launch
-    HIP events."""
+    requests (padded orders). Mixed (FP32 + refinement) and plain FP64. The
+    roofline is the dominant kernel class of one profiled solve of the trace's
+    largest system (per-launch HIP events)."""
     import ctypes
 
     import oracle
     import numpp_amd
+    from numpp_amd import _lib
     from numpp_amd.serve import Server
 
     lib = numpp_amd.require_device()
@@ -625,17 +626,36 @@ def bench_serve(args):
             log(f"{prec} rate {lam:g}/s: {run['solves_per_s']:.1f} solves/s, p50 {run['p50_ms']:.1f} ms, "
                 f"p99 {run['p99_ms']:.1f} ms, berr_max {berr:.2e}, ok {run['ok']}/{R}, "
                 f"fallbacks {run['fallbacks']}, batched {run['batched']}")
-    # kernel classes of one replay (headline precision, lowest rate)
+    # Kernel classes of the trace's largest system (the requests that set
+    # p99), solved once with per-launch HIP events in the headline precision,
+    # padded as the server pads it. Per-launch events are not recorded inside
+    # a whole replay: with four worker threads a profiled replay of the full
+    # trace went silent for minutes, for a cause not yet found.
+    from numpp_amd import linalg as nla
+    from numpp_amd.device import DeviceArray
+
+    big = max((i for i in range(R) if mats[i] is not None), key=lambda i: mats[i].shape[0])
+    nb = mats[big].shape[0]
+    Nb = min(4096, -(-nb // 128) * 128)
+    ap = np.eye(Nb)
+    ap[:nb, :nb] = mats[big]
+    bp = np.zeros((Nb, rhss[big].shape[1]))
+    bp[:nb] = rhss[big]
+    dA, dB = DeviceArray.from_host(ap[None]), DeviceArray.from_host(bp[None])
+    dX = DeviceArray(dB.shape)
+    opts = nla.refine_opts(precs[0], 1e-12, 10, True, 6)
+    nla.gesv_refined_dev(dA, dB, dX, opts=opts)  # warm-up: sizes workspaces
+    _lib.check(lib.npph_stream_sync(None))
     lib.npph_prof_reset()
     lib.npph_prof_enable(1)
-    srv = Server(budget_bytes=32e9, workers=4, max_batch=8, precision=precs[0])
-    srv.replay(arrivals(min(rates)), mats, rhss, keys, keeps)
-    srv.close()
+    nla.gesv_refined_dev(dA, dB, dX, opts=opts)
+    _lib.check(lib.npph_stream_sync(None))
     lib.npph_prof_enable(0)
     prof = prof_report(lib, ctypes)
     lib.npph_prof_reset()
     dom = max(prof, key=lambda k: prof[k]["ms"])
-    roof = roofline_from(prof, dom)
+    roof = roofline_from(prof, dom, {"scope": f"one {precs[0]} refined solve of the trace's largest system, "
+                                              f"n = {nb} padded to {Nb}, {rhss[big].shape[1]} RHS"})
     cpu = None
     if not args.no_cpu:
         # The same trace served offline on one core, in arrival order, until

@@ -478,9 +478,12 @@ int residual_launch(int64_t count, int n, int nrhs, const double* A, const doubl
 
 // Bytes of factor storage for a uniform stack (see RefineFactors).
 size_t refine_factor_bytes(int precision, int64_t count, int n) {
+  // the pieces refine_factor_carve takes, each rounded up to 256 bytes
+  auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
   const size_t nn = (size_t)count * n * n;
   const size_t lu = precision == NPPH_PREC_FP64 ? nn * 8 : nn * 4;
-  return lu + 2 * sizeof(int32_t) * (size_t)count * n + sizeof(int32_t) * count + 8 * (size_t)count + 1024;
+  return al(lu) + al(8 * (size_t)count) + 2 * al(sizeof(int32_t) * (size_t)count * n) +
+         al(sizeof(int32_t) * (size_t)count);
 }
 
 // Carves a RefineFactors out of one caller-owned device buffer of

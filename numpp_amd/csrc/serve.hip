@@ -9,14 +9,18 @@
 //   and every resident factorisation; least-recently-used resident factors that
 //   no running request uses are evicted to make room. A request that cannot fit
 //   even on an empty device fails with NPPH_ERR_BUDGET.
-// - Continuous batching: when a worker takes the head request it also takes
-//   queued one-shot requests of the same order n (up to max_batch, budget
-//   permitting) and solves them as one stack; right-hand sides are padded with
-//   zero columns to the widest request. Every kernel works per system and per
-//   column, so a result is bitwise the same alone and in the mix.
-// - Resident factors: a request with keep = 1 leaves (A, factors) on the device
-//   under its key; a later request with the same key skips the factorisation
-//   and only runs the refined solve with new right-hand sides. A reuse request
+// - Continuous batching: every request is solved at a padded order N
+//   (bucket_order: diag(A, I), B with zero rows). When a worker takes a head
+//   request that factorises, it also takes queued factorising requests of the
+//   same N (up to max_batch, budget permitting; requests that keep their
+//   factors included) and solves them as one stack; right-hand sides are
+//   padded with zero columns to the widest request. Every kernel works per
+//   system and per column and ||A||_inf covers the caller's rows only, so a
+//   result is bitwise the same alone and in the mix.
+// - Resident factors: a request with keep = 1 leaves (padded A, factors) on the
+//   device under its key (a copy of its slice when it ran in a group); a later
+//   request with the same key skips the factorisation and only runs the
+//   refined solve with new right-hand sides. A reuse request
 //   whose key is still being factorised by an earlier request (queued or
 //   running) is held back and queued again as soon as the factors are resident.
 // Host buffers passed to submit must stay valid until wait returns.
@@ -27,6 +31,7 @@
 #include <deque>
 #include <list>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <thread>
 #include <unordered_map>
@@ -42,8 +47,8 @@ using Clock = std::chrono::steady_clock;
 
 struct Resident {
   uint64_t key = 0;
-  int n = 0;
-  double* A = nullptr;  // FP64 copy (residual)
+  int n = 0, N = 0;      // caller's order, padded order
+  double* A = nullptr;  // padded FP64 copy (residual)
   void* fbuf = nullptr;  // factors
   RefineFactors f;
   size_t bytes = 0;
@@ -53,7 +58,7 @@ struct Resident {
 
 struct Request {
   int64_t id = 0;
-  int n = 0, nrhs = 0;
+  int n = 0, N = 0, nrhs = 0;  // N: padded order (bucket_order)
   const double* A = nullptr;
   const double* B = nullptr;
   double* X = nullptr;
@@ -93,9 +98,12 @@ size_t ws_estimate(int n, int nrhs) {
   return nn * 8 + nb * 8 * 3 + nb * 4 * 2 + (size_t)n * 128 * 8 + 64 * 1024;
 }
 
-size_t request_bytes(const Server& sv, int n, int nrhs, bool reuse) {
-  size_t b = 2 * (size_t)n * nrhs * 8 + ws_estimate(n, nrhs);
-  if (!reuse) b += (size_t)n * n * 8 + refine_factor_bytes(sv.opts.precision, 1, n);
+// Device bytes of one request of padded order N with R right-hand-side
+// columns (the group's widest).
+size_t request_bytes(const Server& sv, int N, int R, bool reuse) {
+  size_t b = 2 * (size_t)N * R * 8 + ws_estimate(N, R);
+  // padded A, factors, and the packed staging of A and B
+  if (!reuse) b += (size_t)N * N * 8 + refine_factor_bytes(sv.opts.precision, 1, N) + (size_t)N * (N + R) * 8;
   return b;
 }
 
@@ -175,17 +183,85 @@ struct AsyncBufs {
   ~AsyncBufs() { free_all(); }
 };
 
-// Solves one request with resident factors.
+// Padded order of a request. Requests whose orders share a bucket are
+// factorised together as diag(A, I) of the bucket's order: powers of two up to
+// 64 (the small-order LU's sizes), then multiples of 32 up to 256, of 64 up to
+// 1024 (the recursive LU's leaf width) and of 128 above. Every request is
+// padded, alone or in a group, so its result does not depend on its company.
+int bucket_order(int n) {
+  int N;
+  if (n <= 64) {
+    N = 8;
+    while (N < n) N <<= 1;
+  } else if (n <= 256) {
+    N = (n + 31) & ~31;
+  } else if (n <= 1024) {
+    N = (n + 63) & ~63;
+  } else {
+    N = (n + 127) & ~127;
+  }
+  return std::max(n, std::min(N, max_blocked_n()));
+}
+
+// Pads one system: src holds the caller's n x n matrix (contiguous, as copied
+// from the host); A (N x N FP64) receives diag(A, I), the LU's working copy
+// `lo` (FP64, or FP32 for the mixed precisions) the same values, and
+// ||A||_inf is taken over the caller's n rows only (*anorm starts at zero).
+// One wave per row of the padded matrix.
+template <class LO>
+__global__ __launch_bounds__(256) void pad_prep_kernel(int N, int n, const double* __restrict__ src,
+                                                       double* __restrict__ A, LO* __restrict__ lo, double* anorm) {
+  const int i = (int)(((int64_t)blockIdx.x * 256 + threadIdx.x) >> 6);
+  const int lane = threadIdx.x & 63;
+  if (i >= N) return;
+  const double* __restrict__ srow = src + (int64_t)min(i, n - 1) * n;
+  double* __restrict__ a = A + (int64_t)i * N;
+  LO* __restrict__ o = lo + (int64_t)i * N;
+  double s = 0.0;
+  for (int j = lane; j < N; j += 64) {
+    const bool in = i < n && j < n;
+    const double v = in ? srow[j] : (i == j ? 1.0 : 0.0);
+    s += in ? fabs(v) : 0.0;
+    a[j] = v;
+    o[j] = (LO)v;
+  }
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) s += __shfl_xor(s, m, 64);
+  if (lane == 0 && i < n)
+    atomicMax(reinterpret_cast<unsigned long long*>(anorm), (unsigned long long)__double_as_longlong(s));
+}
+
+// dst (rows x dcols) <- src (n x m, contiguous) in its leading block, zeros
+// elsewhere; with rows = n and dcols = m it is a plain copy.
+__global__ void pad_block_kernel(int rows, int dcols, int n, int m, const double* __restrict__ src,
+                                 double* __restrict__ dst) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (int64_t)rows * dcols) return;
+  const int i = (int)(e / dcols), c = (int)(e - (int64_t)i * dcols);
+  dst[e] = (i < n && c < m) ? src[(int64_t)i * m + c] : 0.0;
+}
+
+// dst (n x m, contiguous) <- the leading n x m block of src (rows at pitch scols).
+__global__ void unpad_block_kernel(int n, int m, int scols, const double* __restrict__ src, double* __restrict__ dst) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (int64_t)n * m) return;
+  const int i = (int)(e / m), c = (int)(e - (int64_t)i * m);
+  dst[e] = src[(int64_t)i * scols + c];
+}
+
+// Solves one request with resident factors (of the request's padded order).
 int serve_reuse(Server& sv, Request* r, Resident* res, hipStream_t s) {
   AsyncBufs mem(s);
   double *dB, *dX, *dE;
   int32_t *dI, *dF;
-  const size_t nb = (size_t)r->n * r->nrhs;
-  NPPH_TRY(mem.get(&dB, nb * 8));
-  NPPH_TRY(mem.get(&dX, nb * 8));
+  const size_t nb = (size_t)r->n * r->nrhs, Nb = (size_t)res->N * r->nrhs;
+  NPPH_TRY(mem.get(&dB, Nb * 8));
+  NPPH_TRY(mem.get(&dX, Nb * 8));
   NPPH_TRY(mem.get(&dE, (size_t)r->nrhs * 8));
   NPPH_TRY(mem.get(&dI, 4));
   NPPH_TRY(mem.get(&dF, 4));
+  // row-major: the caller's n rows are the leading n * nrhs elements
+  if (Nb > nb) NPPH_HIP(hipMemsetAsync(dB + nb, 0, (Nb - nb) * 8, s));
   NPPH_HIP(hipMemcpyAsync(dB, r->B, nb * 8, hipMemcpyHostToDevice, s));
   NPPH_TRY(refine_solve(sv.opts, res->A, res->f, r->nrhs, dB, dX, dE, dI, dF, s));
   NPPH_HIP(hipMemcpyAsync(r->X, dX, nb * 8, hipMemcpyDeviceToHost, s));
@@ -197,67 +273,122 @@ int serve_reuse(Server& sv, Request* r, Resident* res, hipStream_t s) {
   return NPPH_OK;
 }
 
-// Factorises and solves a group of same-order requests as one stack. A
-// single request with keep = 1 hands its (A, factors) to the cache.
-int serve_group(Server& sv, std::vector<Request*>& g, hipStream_t s, Resident** keep_out) {
-  const int n = g[0]->n;
+// Factorises and solves a group of requests of one padded order N as one
+// stack; right-hand sides are padded with zero rows and with zero columns to
+// the widest request. Requests with keep = 1 hand their (A, factors) to the
+// cache: a lone request hands over its buffers, a request in a group gets a
+// copy of its slice, so the group's buffers can go.
+int serve_group(Server& sv, std::vector<Request*>& g, hipStream_t s, std::vector<Resident*>& kept) {
+  const int N = g[0]->N;
   const int64_t k = (int64_t)g.size();
+  const int prec = sv.opts.precision;
   int R = 1;
   for (auto* r : g) R = std::max(R, r->nrhs);
-  const size_t nn = (size_t)n * n, nbR = (size_t)n * R;
+  const size_t NN = (size_t)N * N, NR = (size_t)N * R;
   AsyncBufs mem(s);
   double *dA, *dB, *dX, *dE;
   int32_t *dI, *dF;
   void* fbuf;
-  NPPH_TRY(mem.get(&dA, k * nn * 8));
-  NPPH_TRY(mem.get(&dB, k * nbR * 8));
-  NPPH_TRY(mem.get(&dX, k * nbR * 8));
+  NPPH_TRY(mem.get(&dA, k * NN * 8));
+  NPPH_TRY(mem.get(&dB, k * NR * 8));
+  NPPH_TRY(mem.get(&dX, k * NR * 8));
   NPPH_TRY(mem.get(&dE, k * R * 8));
   NPPH_TRY(mem.get(&dI, k * 4));
   NPPH_TRY(mem.get(&dF, k * 4));
-  NPPH_TRY(mem.get(&fbuf, refine_factor_bytes(sv.opts.precision, k, n)));
+  NPPH_TRY(mem.get(&fbuf, refine_factor_bytes(prec, k, N)));
+  // Host data moves only by contiguous copies (a pitched copy from pageable
+  // host memory is issued row by row): each request's A and B land packed in
+  // a staging buffer and are padded on the device; X goes back the same way.
+  std::vector<size_t> soff(k + 1, 0);
+  for (int64_t i = 0; i < k; ++i)
+    soff[i + 1] = soff[i] + (size_t)g[i]->n * g[i]->n + (size_t)g[i]->n * g[i]->nrhs;
+  double* stage;
+  NPPH_TRY(mem.get(&stage, soff[k] * 8));
+  RefineFactors f;
+  refine_factor_carve(prec, k, N, fbuf, f);
+  NPPH_HIP(hipMemsetAsync(f.anorm, 0, (size_t)k * 8, s));
   for (int64_t i = 0; i < k; ++i) {
     Request* r = g[i];
-    NPPH_HIP(hipMemcpyAsync(dA + i * nn, r->A, nn * 8, hipMemcpyHostToDevice, s));
-    if (r->nrhs == R) {
-      NPPH_HIP(hipMemcpyAsync(dB + i * nbR, r->B, nbR * 8, hipMemcpyHostToDevice, s));
-    } else {
-      NPPH_HIP(hipMemsetAsync(dB + i * nbR, 0, nbR * 8, s));
-      NPPH_HIP(hipMemcpy2DAsync(dB + i * nbR, (size_t)R * 8, r->B, (size_t)r->nrhs * 8, (size_t)r->nrhs * 8, n,
-                                hipMemcpyHostToDevice, s));
-    }
+    const int n = r->n;
+    const size_t nn = (size_t)n * n, nb = (size_t)n * r->nrhs;
+    double* sa = stage + soff[i];
+    double* sb = sa + nn;
+    NPPH_HIP(hipMemcpyAsync(sa, r->A, nn * 8, hipMemcpyHostToDevice, s));
+    NPPH_HIP(hipMemcpyAsync(sb, r->B, nb * 8, hipMemcpyHostToDevice, s));
+    const dim3 grid((unsigned)ceil_div((int64_t)N * 64, 256));
+    if (prec == NPPH_PREC_FP64)
+      hipLaunchKernelGGL(pad_prep_kernel<double>, grid, dim3(256), 0, s, N, n, sa, dA + i * NN, f.lu64 + i * NN,
+                         f.anorm + i);
+    else
+      hipLaunchKernelGGL(pad_prep_kernel<float>, grid, dim3(256), 0, s, N, n, sa, dA + i * NN, f.lu32 + i * NN,
+                         f.anorm + i);
+    hipLaunchKernelGGL(pad_block_kernel, dim3((unsigned)ceil_div((int64_t)NR, 256)), dim3(256), 0, s, N, R, n,
+                       r->nrhs, sb, dB + i * NR);
+    NPPH_HIP(hipGetLastError());
   }
-  RefineFactors f;
-  refine_factor_carve(sv.opts.precision, k, n, fbuf, f);
-  NPPH_TRY(refine_factor(sv.opts, dA, f, s));
+  NPPH_TRY(refine_factor(sv.opts, dA, f, s, /*prepped=*/true));
   NPPH_TRY(refine_solve(sv.opts, dA, f, R, dB, dX, dE, dI, dF, s));
   std::vector<int32_t> iters(k), info(k);
   std::vector<double> berr((size_t)k * R);
   for (int64_t i = 0; i < k; ++i) {
     Request* r = g[i];
-    NPPH_HIP(hipMemcpy2DAsync(r->X, (size_t)r->nrhs * 8, dX + i * nbR, (size_t)R * 8, (size_t)r->nrhs * 8, n,
-                              hipMemcpyDeviceToHost, s));
+    const size_t nb = (size_t)r->n * r->nrhs;
+    double* sx = stage + soff[i] + (size_t)r->n * r->n;  // B's staging, no longer read
+    hipLaunchKernelGGL(unpad_block_kernel, dim3((unsigned)ceil_div((int64_t)nb, 256)), dim3(256), 0, s, r->n,
+                       r->nrhs, R, dX + i * NR, sx);
+    NPPH_HIP(hipGetLastError());
+    NPPH_HIP(hipMemcpyAsync(r->X, sx, nb * 8, hipMemcpyDeviceToHost, s));
   }
   NPPH_HIP(hipMemcpyAsync(berr.data(), dE, berr.size() * 8, hipMemcpyDeviceToHost, s));
   NPPH_HIP(hipMemcpyAsync(iters.data(), dI, k * 4, hipMemcpyDeviceToHost, s));
   NPPH_HIP(hipMemcpyAsync(info.data(), dF, k * 4, hipMemcpyDeviceToHost, s));
+  // resident copies of kept slices (group) or the buffers themselves (alone)
+  const size_t rbytes = NN * 8 + refine_factor_bytes(prec, 1, N);
+  std::vector<std::unique_ptr<Resident>> keep;
+  AsyncBufs kmem(s);
+  for (int64_t i = 0; i < k; ++i) {
+    Request* r = g[i];
+    if (!(r->keep && r->key)) continue;
+    std::unique_ptr<Resident> res(new Resident);
+    res->key = r->key;
+    res->n = r->n;
+    res->N = N;
+    res->bytes = rbytes;
+    if (k == 1) {
+      res->A = dA;
+      res->fbuf = fbuf;
+      res->f = f;
+    } else {
+      NPPH_TRY(kmem.get(&res->A, NN * 8));
+      NPPH_TRY(kmem.get(&res->fbuf, refine_factor_bytes(prec, 1, N)));
+      refine_factor_carve(prec, 1, N, res->fbuf, res->f);
+      const RefineFactors& d = res->f;
+      NPPH_HIP(hipMemcpyAsync(res->A, dA + i * NN, NN * 8, hipMemcpyDeviceToDevice, s));
+      if (prec == NPPH_PREC_FP64)
+        NPPH_HIP(hipMemcpyAsync(d.lu64, f.lu64 + i * NN, NN * 8, hipMemcpyDeviceToDevice, s));
+      else
+        NPPH_HIP(hipMemcpyAsync(d.lu32, f.lu32 + i * NN, NN * 4, hipMemcpyDeviceToDevice, s));
+      NPPH_HIP(hipMemcpyAsync(d.ipiv, f.ipiv + i * N, (size_t)N * 4, hipMemcpyDeviceToDevice, s));
+      NPPH_HIP(hipMemcpyAsync(d.perm, f.perm + i * N, (size_t)N * 4, hipMemcpyDeviceToDevice, s));
+      NPPH_HIP(hipMemcpyAsync(d.info, f.info + i, 4, hipMemcpyDeviceToDevice, s));
+      NPPH_HIP(hipMemcpyAsync(d.anorm, f.anorm + i, 8, hipMemcpyDeviceToDevice, s));
+    }
+    keep.push_back(std::move(res));
+  }
   NPPH_HIP(hipStreamSynchronize(s));
   for (int64_t i = 0; i < k; ++i) {
     g[i]->iters = iters[i];
     g[i]->info = info[i];
     if (g[i]->berr) std::copy(berr.begin() + i * R, berr.begin() + i * R + g[i]->nrhs, g[i]->berr);
   }
-  if (k == 1 && g[0]->keep && g[0]->key != 0) {
-    Resident* res = new Resident;
-    res->key = g[0]->key;
-    res->n = n;
-    res->A = dA;
-    res->fbuf = fbuf;
-    res->f = f;
-    res->bytes = nn * 8 + refine_factor_bytes(sv.opts.precision, 1, n);
-    mem.release(dA);
-    mem.release(fbuf);
-    *keep_out = res;
+  for (auto& res : keep) {
+    kmem.release(res->A);
+    kmem.release(res->fbuf);
+    if (k == 1) {
+      mem.release(dA);
+      mem.release(fbuf);
+    }
+    kept.push_back(res.release());
   }
   mem.free_all();
   NPPH_HIP(hipStreamSynchronize(s));
@@ -281,7 +412,7 @@ void worker_main(Server* sv) {
         Request* h = sv->queue.front();
         auto it = h->key ? sv->cache.find(h->key) : sv->cache.end();
         const bool is_reuse = it != sv->cache.end() && it->second->n == h->n;
-        h->bytes = request_bytes(*sv, h->n, h->nrhs, is_reuse);
+        h->bytes = request_bytes(*sv, is_reuse ? it->second->N : h->N, h->nrhs, is_reuse);
         if (!is_reuse && !h->A && sv->pending_keep.count(h->key)) {
           // its factors are still being computed: hold it until they are resident
           sv->queue.pop_front();
@@ -306,25 +437,38 @@ void worker_main(Server* sv) {
           sv->cv.notify_all();
           continue;
         }
-        if (!make_room(*sv, h->bytes, freed)) { sv->cv.wait(lk); continue; }
+        // a reuse request pins its factors first, so making room cannot evict them
+        if (is_reuse) it->second->refs++;
+        if (!make_room(*sv, h->bytes, freed)) {
+          if (is_reuse) it->second->refs--;
+          sv->cv.wait(lk);
+          continue;
+        }
         sv->queue.pop_front();
         sv->used += h->bytes;
         group.push_back(h);
         if (is_reuse) {
           reuse = it->second;
-          reuse->refs++;
           sv->lru.erase(reuse->lru);
           sv->lru.push_front(reuse->key);
           reuse->lru = sv->lru.begin();
-        } else if (!(h->keep && h->key)) {
-          // coalesce queued one-shot requests of the same order
+        } else {
+          // coalesce queued requests that factorise, in the same padded order
+          // (requests that keep their factors included); the group's bytes
+          // grow with its widest right-hand side
+          int R = h->nrhs;
+          size_t gbytes = h->bytes;
           for (auto q = sv->queue.begin(); q != sv->queue.end() && (int)group.size() < sv->cfg.max_batch;) {
             Request* c = *q;
-            if (c->n == h->n && c->A && !(c->keep && c->key) && !(c->key && sv->cache.count(c->key))) {
-              const size_t b = request_bytes(*sv, c->n, c->nrhs, false);
-              if (sv->used + b <= sv->budget) {
-                c->bytes = b;
-                sv->used += b;
+            auto cr = c->key ? sv->cache.find(c->key) : sv->cache.end();
+            const bool c_reuse = cr != sv->cache.end() && cr->second->n == c->n;
+            if (c->N == h->N && c->A && !c_reuse) {
+              const int R2 = std::max(R, c->nrhs);
+              const size_t total = request_bytes(*sv, h->N, R2, false) * (group.size() + 1);
+              if (sv->used + (total - gbytes) <= sv->budget) {
+                sv->used += total - gbytes;
+                gbytes = total;
+                R = R2;
                 group.push_back(c);
                 q = sv->queue.erase(q);
                 continue;
@@ -332,6 +476,7 @@ void worker_main(Server* sv) {
             }
             ++q;
           }
+          for (auto* r : group) r->bytes = gbytes / group.size();
         }
         sv->peak = std::max(sv->peak, sv->used);
         break;
@@ -343,9 +488,9 @@ void worker_main(Server* sv) {
     const auto t0 = Clock::now();
     for (auto* r : group) { r->t_start = t0; r->batched = group.size() > 1; }
     int st;
-    Resident* kept = nullptr;
+    std::vector<Resident*> kept;
     if (reuse) st = serve_reuse(*sv, group[0], reuse, s);
-    else st = serve_group(*sv, group, s, &kept);
+    else st = serve_group(*sv, group, s, kept);
     {
       std::lock_guard<std::mutex> lk(sv->mu);
       if (reuse) reuse->refs--;
@@ -358,8 +503,8 @@ void worker_main(Server* sv) {
         if (r->iters < 0) ++sv->fallbacks;
       }
       if (group.size() > 1) sv->batched += (int64_t)group.size();
-      if (kept) {
-        auto old = sv->cache.find(kept->key);
+      for (auto& kr : kept) {
+        auto old = sv->cache.find(kr->key);
         if (old != sv->cache.end()) {
           // replace an older factorisation under the same key (not in use: refs checked)
           Resident* o = old->second;
@@ -370,20 +515,21 @@ void worker_main(Server* sv) {
             freed.push_back(o);
           }
         }
-        if (!sv->cache.count(kept->key)) {
-          sv->lru.push_front(kept->key);
-          kept->lru = sv->lru.begin();
-          sv->cache[kept->key] = kept;
-          sv->used += kept->bytes;
+        if (!sv->cache.count(kr->key)) {
+          sv->lru.push_front(kr->key);
+          kr->lru = sv->lru.begin();
+          sv->cache[kr->key] = kr;
+          sv->used += kr->bytes;
           sv->peak = std::max(sv->peak, sv->used);
-          kept = nullptr;
+          kr = nullptr;
         }
       }
       for (auto* r : group)
         if (r->A && r->keep && r->key) release_pending(*sv, r->key);
       sv->cv.notify_all();
     }
-    if (kept) free_resident(kept, s);
+    for (Resident* kr : kept)
+      if (kr) free_resident(kr, s);
     for (Resident* r : freed) free_resident(r, s);
     (void)hipStreamSynchronize(s);
   }
@@ -458,6 +604,7 @@ int npph_server_submit(void* srv, int64_t n, int64_t nrhs, const double* A, cons
   }
   Request* r = new Request;
   r->n = (int)n;
+  r->N = bucket_order((int)n);
   r->nrhs = (int)nrhs;
   r->A = A;
   r->B = B;

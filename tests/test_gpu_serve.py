@@ -111,3 +111,34 @@ def test_replay_poisson(hip):
     assert np.all(out["latency_ms"] > 0)
     for (a, b), x in zip(syss, out["x"]):
         assert np.all(oracle.backward_error(a, b, x) <= 1e-12)
+
+
+def test_bucket_coalescing_with_kept_factors(hip):
+    # A burst of requests whose orders share one padded bucket (97..128 -> 128),
+    # all keeping their factors: one worker takes the first, the rest queue up
+    # behind it and are factorised as one padded stack. Each kept slice then
+    # serves new right-hand sides, and every result equals the request alone.
+    rng = np.random.default_rng(7)
+    ns = [128, 97, 100, 110, 120, 101, 99, 127]
+    mats = [cond_matrix(rng, n, 1e4) for n in ns]
+    bs = [rng.standard_normal((n, 1 + i % 3)) for i, n in enumerate(ns)]
+    srv = Server(budget_bytes=8e9, workers=1, max_batch=8, precision="fp32")
+    ids = [srv.submit(a, b, key=100 + i, keep=True) for i, (a, b) in enumerate(zip(mats, bs))]
+    res = [srv.wait(i) for i in ids]
+    st = srv.stats()
+    assert all(r["status"] == 0 for r in res)
+    assert st["batched"] >= 2 and st["resident"] == len(ns), st
+    for a, b, r in zip(mats, bs, res):
+        assert np.all(oracle.backward_error(a, b, r["x"]) <= 1e-12)
+        assert np.all(r["berr"] <= 1e-12)
+    for i, a in enumerate(mats):
+        b = rng.standard_normal((a.shape[0], 2))
+        r = srv.wait(srv.submit(None, b, key=100 + i))
+        assert r["status"] == 0
+        assert np.all(oracle.backward_error(a, b, r["x"]) <= 1e-12)
+    srv.close()
+    for i in (1, 4, 7):
+        s1 = Server(budget_bytes=8e9, workers=1, max_batch=1, precision="fp32")
+        r = s1.wait(s1.submit(mats[i], bs[i]))
+        s1.close()
+        np.testing.assert_array_equal(r["x"], res[i]["x"])
