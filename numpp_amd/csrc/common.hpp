@@ -70,6 +70,7 @@ struct WsSet;
 WsSet* ws_create(hipStream_t owner);
 void ws_destroy(WsSet* w);
 void ws_bind(WsSet* w);
+WsSet* ws_bound();  // the set bound on the calling thread (nullptr: none)
 bool ws_private();  // the calling thread has a private set bound
 // Frees the per-stream set of `s` (after synchronising s).
 void ws_release_stream(hipStream_t s);

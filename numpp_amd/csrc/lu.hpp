@@ -99,6 +99,54 @@ int refine_factor(const npph_refine_opts& o, const double* A, RefineFactors& f, 
 // residual) until every column meets o.tol; B and X must not alias.
 int refine_solve(const npph_refine_opts& o, const double* A, const RefineFactors& f, int nrhs, const double* B,
                  double* X, double* berr, int32_t* iters_out, int32_t* info_out, hipStream_t s);
+// One refined solve as resumable steps, for callers that issue the sweeps of
+// several stacks before waiting on any (refine_solve = begin, loop, end):
+//   refine_begin  binds the bookkeeping (the bound workspace set's kWsNorm,
+//                 kWsR, kWsD slots) and issues sweep 0 (FP64: the whole solve);
+//   refine_steps  issues `budget` test + correction iterations and the next
+//                 test, with no host read-back (capturable in a graph);
+//   refine_loop   continues with a host read-back per iteration until no
+//                 system iterates; n_fb = systems stopped without converging;
+//   refine_end    FP64 fallback of those (when enabled and n_fb > 0) and the
+//                 outputs (with n_fb = 0: the outputs only; callable again).
+// The device state lives in workspace slots: nothing else may use the same
+// set between the steps of one run.
+// Matrices of a bucket kept at their own orders (n <= 256): slot k holds
+// system ids[k], whose nn[id] x nn[id] matrix starts at A + aoff[id]; the
+// padded system is diag(A, I).
+struct PackedA {
+  const int32_t* ids = nullptr;
+  const int32_t* nn = nullptr;
+  const int64_t* aoff = nullptr;
+  const double* A = nullptr;
+};
+
+struct RefineRun {
+  npph_refine_opts o{};
+  const double* A = nullptr;  // uniform padded stack, unless var.A is set
+  PackedA var;
+  RefineFactors f;
+  int nrhs = 0;
+  const double* B = nullptr;
+  double* X = nullptr;
+  double* berr_out = nullptr;
+  int32_t* iters_out = nullptr;
+  int32_t* info_out = nullptr;
+  // set by refine_begin
+  int64_t count = 0;
+  int n = 0;
+  double *bmax = nullptr, *rmax = nullptr, *xmax = nullptr, *scale = nullptr, *berr_p = nullptr, *prev = nullptr;
+  double* R = nullptr;
+  float* Dlo = nullptr;
+  int32_t *info = nullptr, *active = nullptr, *iters = nullptr, *mask = nullptr, *ctr = nullptr;
+  int it = 0;               // next loop iteration
+  bool eta_issued = false;  // the test of iteration `it` is already queued
+  bool done = false;
+};
+int refine_begin(RefineRun& r, hipStream_t s);
+int refine_steps(RefineRun& r, int budget, hipStream_t s);
+int refine_loop(RefineRun& r, int32_t* n_fb, hipStream_t s);
+int refine_end(RefineRun& r, int32_t n_fb, hipStream_t s);
 int refined_impl(const npph_refine_opts& o, int64_t count, int n, int nrhs, const double* A, const double* B,
                  double* X, double* berr, int32_t* iters_out, int32_t* info_out, hipStream_t s);
 int gesv_dev_impl(int64_t count, int n, int nrhs, double* A, int64_t strideA, double* B, int64_t strideB,

@@ -33,8 +33,7 @@ int gesv_dev_impl(int64_t count, int n, int nrhs, double* A, int64_t strideA, do
   // Singular systems (info != 0) are skipped by getrs; their B content on
   // return is unspecified, as the reference raises instead of returning one.
   if (strideB == sx) {
-    NPPH_HIP(hipMemcpy2DAsync(B, sizeof(double) * sx, X, sizeof(double) * sx, sizeof(double) * sx, count,
-                              hipMemcpyDeviceToDevice, s));
+    NPPH_HIP(hipMemcpyAsync(B, X, sizeof(double) * sx * count, hipMemcpyDeviceToDevice, s));
   } else {
     for (int64_t i = 0; i < count; ++i)
       NPPH_HIP(hipMemcpyAsync(B + i * strideB, X + i * sx, sizeof(double) * sx, hipMemcpyDeviceToDevice, s));

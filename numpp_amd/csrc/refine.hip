@@ -367,6 +367,118 @@ __global__ __launch_bounds__(256) void residual_small_kernel(int n, int nrhs, co
   }
 }
 
+// residual_small_kernel for a bucket of systems kept at their own order: the
+// workgroup's system (bucket slot `mat`, system ids[mat]) has its n x n matrix
+// packed at A + aoff[id] (row pitch n), while X, B and R are the bucket's
+// padded N x nrhs blocks. The padded system is diag(A, I), so rows i >= n give
+// r_i = b_i - x_i and rows i < n sum over k < n only.
+__global__ __launch_bounds__(256) void residual_var_kernel(int N, int nrhs, const int32_t* ids, const int32_t* nn,
+                                                           const int64_t* aoff, const double* Apk, const double* Xb,
+                                                           const double* Bb, double* Rb, double* rmax, double* xmax,
+                                                           const int32_t* active) {
+  constexpr int KU = 4;
+  const int64_t mat = blockIdx.x;
+  if (active && active[mat] != 1) return;
+  const int id = ids[mat], n = nn[id];
+  const int c0 = blockIdx.y * 16, nc = min(16, nrhs - c0);
+  const int64_t per = (int64_t)N * nrhs;
+  const double* __restrict__ A = Apk + aoff[id];
+  const double* __restrict__ X = Xb + mat * per;
+  const double* __restrict__ B = Bb + mat * per;
+  double* __restrict__ R = Rb + mat * per;
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6, lr = lane & 15, lk = lane >> 4;
+  __shared__ double xs[RSM_N][16];
+  __shared__ double red_r[4][16], red_x[4][16];
+  const int np = (N + 15) & ~15;
+  double xm = 0.0;
+  for (int e = t; e < np * 16; e += 256) {
+    const int k = e >> 4, j = e & 15;
+    const double v = (k < N && j < nc) ? X[(int64_t)k * nrhs + c0 + j] : 0.0;
+    xs[k][j] = v;
+    xm = fmax(xm, fabs(v));
+  }
+  xm = fmax(xm, __shfl_xor(xm, 16, 64));
+  xm = fmax(xm, __shfl_xor(xm, 32, 64));
+  if (lane < 16) red_x[wave][lane] = xm;
+  __syncthreads();
+  // 16-byte loads need an even pitch and a 16-byte aligned start
+  const bool vec = (n % 2 == 0) && n >= 2 && ((reinterpret_cast<uintptr_t>(A) & 15) == 0);
+  double rm = 0.0;
+  for (int r0 = wave * 16; r0 < N; r0 += 64) {
+    v4d acc = {0.0, 0.0, 0.0, 0.0};
+    if (r0 < n) {
+      const int row = min(r0 + lr, n - 1);
+      const bool live = r0 + lr < n;
+      const double* __restrict__ ar = A + (int64_t)row * n;
+      for (int k0 = 0; k0 < n; k0 += 16 * KU) {
+        double a[KU][4];
+#pragma unroll
+        for (int u = 0; u < KU; ++u) {
+          const int k = k0 + 16 * u + 4 * lk;
+          if (vec) {
+            const double2 p0 = *reinterpret_cast<const double2*>(ar + min(k, n - 2));
+            const double2 p1 = *reinterpret_cast<const double2*>(ar + min(k + 2, n - 2));
+            a[u][0] = k < n ? p0.x : 0.0;
+            a[u][1] = k + 1 < n ? p0.y : 0.0;
+            a[u][2] = k + 2 < n ? p1.x : 0.0;
+            a[u][3] = k + 3 < n ? p1.y : 0.0;
+          } else {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) a[u][q] = k + q < n ? ar[min(k + q, n - 1)] : 0.0;
+          }
+#pragma unroll
+          for (int q = 0; q < 4; ++q) a[u][q] = live ? a[u][q] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < KU; ++u) {
+          const int kb = k0 + 16 * u + 4 * lk;
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const int kx = kb + q < np ? kb + q : 0;
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u][q], xs[kx][lr], acc, 0, 0, 0);
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int rr = r0 + lk + 4 * v;
+      if (rr < N && lr < nc) {
+        const double ax = rr < n ? acc[v] : xs[rr][lr];
+        const double r = B[(int64_t)rr * nrhs + c0 + lr] - ax;
+        R[(int64_t)rr * nrhs + c0 + lr] = r;
+        rm = fmax(rm, fabs(r));
+      }
+    }
+  }
+  rm = fmax(rm, __shfl_xor(rm, 16, 64));
+  rm = fmax(rm, __shfl_xor(rm, 32, 64));
+  if (lane < 16) red_r[wave][lane] = rm;
+  __syncthreads();
+  if (t < nc) {
+    double r = red_r[0][t], x = red_x[0][t];
+#pragma unroll
+    for (int w = 1; w < 4; ++w) {
+      r = fmax(r, red_r[w][t]);
+      x = fmax(x, red_x[w][t]);
+    }
+    rmax[mat * nrhs + c0 + t] = r;
+    xmax[mat * nrhs + c0 + t] = x;
+  }
+}
+
+// diag(A_id, I) of order N for each bucket slot, from the packed matrices.
+__global__ void pad_var_kernel(int64_t count, int N, const int32_t* ids, const int32_t* nn, const int64_t* aoff,
+                               const double* Apk, double* out) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t per = (int64_t)N * N;
+  if (e >= count * per) return;
+  const int64_t mat = e / per, w = e - mat * per;
+  const int i = (int)(w / N), j = (int)(w - (int64_t)i * N);
+  const int id = ids[mat], n = nn[id];
+  out[e] = (i < n && j < n) ? Apk[aoff[id] + (int64_t)i * n + j] : (i == j ? 1.0 : 0.0);
+}
+
 // Convergence test per system (one thread per system). Records eta per column,
 // decides active (1 = continue, 0 = converged, 2 = stopped without
 // convergence), computes the next sweep's scales, clears the max buffers and
@@ -541,12 +653,100 @@ int refine_factor(const npph_refine_opts& o, const double* A, RefineFactors& f, 
   return blocked_getrf_perm<float>(count, n, f.lu32, (int64_t)n * n, f.ipiv, f.perm, f.info, terms, nullptr, s);
 }
 
-int refine_solve(const npph_refine_opts& o, const double* A, const RefineFactors& f, int nrhs, const double* B,
-                 double* X, double* berr, int32_t* iters_out, int32_t* info_out, hipStream_t s) {
-  const int64_t count = f.count;
-  const int n = f.n;
-  if (count <= 0 || n <= 0 || nrhs <= 0) return NPPH_OK;
-  const int64_t per_a = (int64_t)n * n, per_b = (int64_t)n * nrhs;
+namespace {
+
+// Plain FP64 solve of the systems selected by `mask` (all when masked is
+// false), from a fresh FP64 factorisation of A.
+int fp64_solve(RefineRun& r, bool masked, hipStream_t s) {
+  const int64_t count = r.count, per_a = (int64_t)r.n * r.n, per_b = (int64_t)r.n * r.nrhs;
+  const int64_t nb_all = count * per_b;
+  double *Aw, *Bw;
+  int32_t *ipiv, *info64;
+  NPPH_TRY(workspace(kWsA, sizeof(double) * (size_t)(count * per_a), (void**)&Aw));
+  NPPH_TRY(workspace(kWsB, sizeof(double) * (size_t)nb_all, (void**)&Bw));
+  NPPH_TRY(workspace(kWsPiv, sizeof(int32_t) * (size_t)(count * r.n), (void**)&ipiv));
+  NPPH_TRY(workspace(kWsInfo, sizeof(int32_t) * (size_t)count, (void**)&info64));
+  if (r.var.A)
+    hipLaunchKernelGGL(pad_var_kernel, g1(count * per_a), dim3(256), 0, s, count, r.n, r.var.ids, r.var.nn,
+                       r.var.aoff, r.var.A, Aw);
+  else
+    NPPH_HIP(hipMemcpyAsync(Aw, r.A, sizeof(double) * count * per_a, hipMemcpyDeviceToDevice, s));
+  NPPH_HIP(hipMemcpyAsync(Bw, r.B, sizeof(double) * nb_all, hipMemcpyDeviceToDevice, s));
+  NPPH_TRY(gesv_dev_impl(count, r.n, r.nrhs, Aw, per_a, Bw, per_b, ipiv, info64, s));
+  if (masked) {
+    hipLaunchKernelGGL(copy_masked_kernel, g1(nb_all), dim3(256), 0, s, count, per_b, Bw, r.X, r.mask);
+    hipLaunchKernelGGL(copy_masked_int_kernel, g1(count), dim3(256), 0, s, count, info64, r.info, r.mask);
+  } else {
+    NPPH_HIP(hipMemcpyAsync(r.X, Bw, sizeof(double) * nb_all, hipMemcpyDeviceToDevice, s));
+    NPPH_HIP(hipMemcpyAsync(r.info, info64, sizeof(int32_t) * count, hipMemcpyDeviceToDevice, s));
+  }
+  NPPH_HIP(hipGetLastError());
+  return NPPH_OK;
+}
+
+// R = B - A X of the run's systems (A uniform, or packed per system).
+int residual_run(RefineRun& r, hipStream_t s) {
+  if (!r.var.A) return residual_launch(r.count, r.n, r.nrhs, r.A, r.X, r.B, r.R, r.rmax, r.xmax, r.active, s);
+  if (r.n > RSM_N || ceil_div(r.nrhs, 16) > 65535) {
+    set_error("refined solve: packed matrices need n <= %d", RSM_N);
+    return NPPH_ERR_INVALID;
+  }
+  ProfScope ps(kProfResidual, 2.0 * r.count * r.n * r.n * r.nrhs,
+               8.0 * r.count * ((double)r.n * r.n + 3.0 * r.n * r.nrhs), s);
+  hipLaunchKernelGGL(residual_var_kernel, dim3((unsigned)r.count, (unsigned)ceil_div(r.nrhs, 16)), dim3(256), 0, s,
+                     r.n, r.nrhs, r.var.ids, r.var.nn, r.var.aoff, r.var.A, r.X, r.B, r.R, r.rmax, r.xmax, r.active);
+  NPPH_HIP(hipGetLastError());
+  return NPPH_OK;
+}
+
+// eta of the systems flagged active (1), without further sweeps.
+int report(RefineRun& r, hipStream_t s) {
+  NPPH_TRY(residual_run(r, s));
+  hipLaunchKernelGGL(eta_kernel, g1(r.count), dim3(256), 0, s, r.count, r.nrhs, INFINITY, 0, r.f.anorm, r.bmax,
+                     r.rmax, r.xmax, r.scale, r.prev, r.berr_p, r.active, r.iters, r.ctr, nullptr);
+  NPPH_HIP(hipGetLastError());
+  return NPPH_OK;
+}
+
+int max_iter_of(const npph_refine_opts& o) { return o.max_iter > 0 ? o.max_iter : 10; }
+
+// Residual and convergence test of loop iteration `it`: systems that meet
+// the tolerance (or have run max_iter sweeps, or stagnate) leave state 1;
+// ctr[0] receives the number still iterating, ctr[1] accumulates those
+// stopped without convergence.
+int eta_step(RefineRun& r, hipStream_t s) {
+  NPPH_TRY(residual_run(r, s));
+  hipLaunchKernelGGL(zero_int_kernel, dim3(1), dim3(64), 0, s, r.ctr, 1);
+  hipLaunchKernelGGL(eta_kernel, g1(r.count), dim3(256), 0, s, r.count, r.nrhs, r.o.tol, max_iter_of(r.o), r.f.anorm,
+                     r.bmax, r.rmax, r.xmax, r.scale, r.prev, r.berr_p, r.active, r.iters, r.ctr, r.ctr + 1);
+  NPPH_HIP(hipGetLastError());
+  return NPPH_OK;
+}
+
+// The correction of loop iteration `it` for the systems still iterating
+// (none past max_iter).
+int corr_step(RefineRun& r, int it, hipStream_t s) {
+  if (it > max_iter_of(r.o)) return NPPH_OK;
+  NPPH_TRY(getrs_refine(r.count, r.n, r.nrhs, r.f.lu32, (int64_t)r.n * r.n, r.f.perm, r.R, r.scale, r.Dlo, r.X, 0,
+                        r.active, s));
+  hipLaunchKernelGGL(count_sweep_kernel, g1(r.count), dim3(256), 0, s, r.count, r.active, r.iters);
+  NPPH_HIP(hipGetLastError());
+  return NPPH_OK;
+}
+
+}  // namespace
+
+int refine_begin(RefineRun& r, hipStream_t s) {
+  const int64_t count = r.f.count;
+  const int n = r.f.n;
+  r.count = count;
+  r.n = n;
+  r.it = 0;
+  r.eta_issued = false;
+  r.done = count <= 0 || n <= 0 || r.nrhs <= 0;
+  if (r.done) return NPPH_OK;
+  const int nrhs = r.nrhs;
+  const int64_t per_b = (int64_t)n * nrhs;
   const int64_t nb_all = count * per_b, nk = count * nrhs;
 
   // Bookkeeping arrays share one workspace slot.
@@ -554,113 +754,115 @@ int refine_solve(const npph_refine_opts& o, const double* A, const RefineFactors
   const size_t int_cnt = (size_t)(4 * count + 2);   // info, active, iters, mask, counters
   char* misc;
   NPPH_TRY(workspace(kWsNorm, dbl_cnt * 8 + int_cnt * 4 + 64, (void**)&misc));
-  double* bmax = (double*)misc;
-  double* rmax = bmax + nk;
-  double* xmax = rmax + nk;
-  double* scale = xmax + nk;
-  double* berr_ws = scale + nk;
-  double* prev = berr_ws + nk;
-  int32_t* info = (int32_t*)(prev + count);
-  int32_t* active = info + count;
-  int32_t* iters = active + count;
-  int32_t* mask = iters + count;
-  int32_t* ctr = mask + count;  // [0] = active count, [1] = fallback count
-  double* berr_p = berr ? berr : berr_ws;
-  double* R;
-  NPPH_TRY(workspace(kWsR, sizeof(double) * (size_t)nb_all, (void**)&R));
-  const double* anorm = f.anorm;
+  r.bmax = (double*)misc;
+  r.rmax = r.bmax + nk;
+  r.xmax = r.rmax + nk;
+  r.scale = r.xmax + nk;
+  double* berr_ws = r.scale + nk;
+  r.prev = berr_ws + nk;
+  r.info = (int32_t*)(r.prev + count);
+  r.active = r.info + count;
+  r.iters = r.active + count;
+  r.mask = r.iters + count;
+  r.ctr = r.mask + count;  // [0] = active count, [1] = fallback count
+  r.berr_p = r.berr_out ? r.berr_out : berr_ws;
+  NPPH_TRY(workspace(kWsR, sizeof(double) * (size_t)nb_all, (void**)&r.R));
 
-  hipLaunchKernelGGL(fill_kernel, g1(3 * nk), dim3(256), 0, s, bmax, (int64_t)(3 * nk), 0.0);
-  hipLaunchKernelGGL(fill_kernel, g1(count), dim3(256), 0, s, prev, count, (double)INFINITY);
-  hipLaunchKernelGGL(fill_kernel, g1(nk), dim3(256), 0, s, berr_p, nk, (double)NAN);
-  hipLaunchKernelGGL(zero_int_kernel, dim3(1), dim3(64), 0, s, ctr, 2);
-  hipLaunchKernelGGL(bnorm_kernel, dim3((unsigned)count), dim3(256), 0, s, n, nrhs, B, bmax);
-  NPPH_HIP(hipMemcpyAsync(info, f.info, sizeof(int32_t) * count, hipMemcpyDeviceToDevice, s));
+  hipLaunchKernelGGL(fill_kernel, g1(3 * nk), dim3(256), 0, s, r.bmax, (int64_t)(3 * nk), 0.0);
+  hipLaunchKernelGGL(fill_kernel, g1(count), dim3(256), 0, s, r.prev, count, (double)INFINITY);
+  hipLaunchKernelGGL(fill_kernel, g1(nk), dim3(256), 0, s, r.berr_p, nk, (double)NAN);
+  hipLaunchKernelGGL(zero_int_kernel, dim3(1), dim3(64), 0, s, r.ctr, 2);
+  hipLaunchKernelGGL(bnorm_kernel, dim3((unsigned)count), dim3(256), 0, s, n, nrhs, r.B, r.bmax);
+  NPPH_HIP(hipMemcpyAsync(r.info, r.f.info, sizeof(int32_t) * count, hipMemcpyDeviceToDevice, s));
   NPPH_HIP(hipGetLastError());
 
-  // Plain FP64 solve of the systems selected by `mask` (all when masked is
-  // false), from a fresh FP64 factorisation of A.
-  auto fp64_solve = [&](bool masked) -> int {
-    double *Aw, *Bw;
-    int32_t *ipiv, *info64;
-    NPPH_TRY(workspace(kWsA, sizeof(double) * (size_t)(count * per_a), (void**)&Aw));
-    NPPH_TRY(workspace(kWsB, sizeof(double) * (size_t)nb_all, (void**)&Bw));
-    NPPH_TRY(workspace(kWsPiv, sizeof(int32_t) * (size_t)(count * n), (void**)&ipiv));
-    NPPH_TRY(workspace(kWsInfo, sizeof(int32_t) * (size_t)count, (void**)&info64));
-    NPPH_HIP(hipMemcpyAsync(Aw, A, sizeof(double) * count * per_a, hipMemcpyDeviceToDevice, s));
-    NPPH_HIP(hipMemcpyAsync(Bw, B, sizeof(double) * nb_all, hipMemcpyDeviceToDevice, s));
-    NPPH_TRY(gesv_dev_impl(count, n, nrhs, Aw, per_a, Bw, per_b, ipiv, info64, s));
-    if (masked) {
-      hipLaunchKernelGGL(copy_masked_kernel, g1(nb_all), dim3(256), 0, s, count, per_b, Bw, X, mask);
-      hipLaunchKernelGGL(copy_masked_int_kernel, g1(count), dim3(256), 0, s, count, info64, info, mask);
-    } else {
-      NPPH_HIP(hipMemcpyAsync(X, Bw, sizeof(double) * nb_all, hipMemcpyDeviceToDevice, s));
-      NPPH_HIP(hipMemcpyAsync(info, info64, sizeof(int32_t) * count, hipMemcpyDeviceToDevice, s));
-    }
-    NPPH_HIP(hipGetLastError());
+  if (r.o.precision == NPPH_PREC_FP64) {
+    NPPH_TRY(blocked_getrs_perm<double>(count, n, nrhs, r.f.lu64, (int64_t)n * n, r.f.perm, r.B, per_b, r.X, per_b,
+                                        r.f.info, nullptr, s));
+    hipLaunchKernelGGL(init_state_kernel, g1(count), dim3(256), 0, s, count, r.info, r.active, r.iters, nullptr);
+    NPPH_TRY(report(r, s));
+    r.done = true;
     return NPPH_OK;
-  };
-  // eta of the systems flagged active (1), without further sweeps.
-  auto report = [&]() -> int {
-    NPPH_TRY(residual_launch(count, n, nrhs, A, X, B, R, rmax, xmax, active, s));
-    hipLaunchKernelGGL(eta_kernel, g1(count), dim3(256), 0, s, count, nrhs, INFINITY, 0, anorm, bmax, rmax, xmax,
-                       scale, prev, berr_p, active, iters, ctr, nullptr);
-    NPPH_HIP(hipGetLastError());
-    return NPPH_OK;
-  };
-
-  if (o.precision == NPPH_PREC_FP64) {
-    NPPH_TRY(blocked_getrs_perm<double>(count, n, nrhs, f.lu64, per_a, f.perm, B, per_b, X, per_b, f.info, nullptr, s));
-    hipLaunchKernelGGL(init_state_kernel, g1(count), dim3(256), 0, s, count, info, active, iters, nullptr);
-    NPPH_TRY(report());
-  } else {
-    float* Dlo;
-    NPPH_TRY(workspace(kWsD, sizeof(float) * (size_t)nb_all, (void**)&Dlo));
-    FlagSlot* fs;
-    NPPH_TRY(flag_slot(&fs));
-    hipLaunchKernelGGL(init_state_kernel, g1(count), dim3(256), 0, s, count, info, active, iters, ctr + 1);
-    hipLaunchKernelGGL(scale_from_bmax_kernel, g1(nk), dim3(256), 0, s, nk, bmax, scale);
-    // Sweep 0 (x = 0, r = b).
-    NPPH_TRY(getrs_refine(count, n, nrhs, f.lu32, per_a, f.perm, B, scale, Dlo, X, 1, active, s));
-    hipLaunchKernelGGL(count_sweep_kernel, g1(count), dim3(256), 0, s, count, active, iters);
-    NPPH_HIP(hipGetLastError());
-    const int max_iter = o.max_iter > 0 ? o.max_iter : 10;
-    // Each test's counts (systems still active, systems stopped so far) come
-    // back through pinned memory while the next sweep is already queued: that
-    // sweep only touches systems still active, so when none is left it changes
-    // nothing, and the device never idles for the host's round trip.
-    for (int it = 0;; ++it) {
-      NPPH_TRY(residual_launch(count, n, nrhs, A, X, B, R, rmax, xmax, active, s));
-      hipLaunchKernelGGL(zero_int_kernel, dim3(1), dim3(64), 0, s, ctr, 1);
-      hipLaunchKernelGGL(eta_kernel, g1(count), dim3(256), 0, s, count, nrhs, o.tol, max_iter, anorm, bmax, rmax,
-                         xmax, scale, prev, berr_p, active, iters, ctr, ctr + 1);
-      NPPH_HIP(hipGetLastError());
-      NPPH_HIP(hipMemcpyAsync(fs->host, ctr, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, s));
-      NPPH_HIP(hipEventRecord(fs->ev, s));
-      // eta stops every system after max_iter sweeps, so the counts reach
-      // zero by then; the bound only guards the loop
-      const bool more = it <= max_iter;
-      if (more) {
-        NPPH_TRY(getrs_refine(count, n, nrhs, f.lu32, per_a, f.perm, R, scale, Dlo, X, 0, active, s));
-        hipLaunchKernelGGL(count_sweep_kernel, g1(count), dim3(256), 0, s, count, active, iters);
-        NPPH_HIP(hipGetLastError());
-      }
-      NPPH_HIP(hipEventSynchronize(fs->ev));
-      if (fs->host[0] == 0 || !more) break;
-    }
-    const int32_t n_fb = fs->host[1];
-    if (o.fallback_fp64) {
-      if (n_fb > 0) {
-        hipLaunchKernelGGL(fallback_mask_kernel, g1(count), dim3(256), 0, s, count, active, mask, iters);
-        NPPH_TRY(fp64_solve(true));
-        NPPH_HIP(hipMemcpyAsync(active, mask, sizeof(int32_t) * count, hipMemcpyDeviceToDevice, s));
-        NPPH_TRY(report());
-      }
-    }
   }
-  if (iters_out) NPPH_HIP(hipMemcpyAsync(iters_out, iters, sizeof(int32_t) * count, hipMemcpyDeviceToDevice, s));
-  if (info_out) NPPH_HIP(hipMemcpyAsync(info_out, info, sizeof(int32_t) * count, hipMemcpyDeviceToDevice, s));
+  NPPH_TRY(workspace(kWsD, sizeof(float) * (size_t)nb_all, (void**)&r.Dlo));
+  hipLaunchKernelGGL(init_state_kernel, g1(count), dim3(256), 0, s, count, r.info, r.active, r.iters, r.ctr + 1);
+  hipLaunchKernelGGL(scale_from_bmax_kernel, g1(nk), dim3(256), 0, s, nk, r.bmax, r.scale);
+  // Sweep 0 (x = 0, r = b).
+  NPPH_TRY(getrs_refine(count, n, nrhs, r.f.lu32, (int64_t)n * n, r.f.perm, r.B, r.scale, r.Dlo, r.X, 1, r.active,
+                        s));
+  hipLaunchKernelGGL(count_sweep_kernel, g1(count), dim3(256), 0, s, count, r.active, r.iters);
+  NPPH_HIP(hipGetLastError());
   return NPPH_OK;
+}
+
+int refine_steps(RefineRun& r, int budget, hipStream_t s) {
+  if (r.done) return NPPH_OK;
+  for (int k = 0; k < budget; ++k) {
+    if (!r.eta_issued) NPPH_TRY(eta_step(r, s));
+    r.eta_issued = false;
+    NPPH_TRY(corr_step(r, r.it, s));
+    ++r.it;
+  }
+  NPPH_TRY(eta_step(r, s));
+  r.eta_issued = true;
+  return NPPH_OK;
+}
+
+int refine_loop(RefineRun& r, int32_t* n_fb, hipStream_t s) {
+  *n_fb = 0;
+  if (r.done) return NPPH_OK;
+  FlagSlot* fs;
+  NPPH_TRY(flag_slot(&fs));
+  // Each test's counts (systems still active, systems stopped so far) come
+  // back through pinned memory while the next sweep is already queued: that
+  // sweep only touches systems still active, so when none is left it changes
+  // nothing, and the device never idles for the host's round trip.
+  for (;;) {
+    if (!r.eta_issued) NPPH_TRY(eta_step(r, s));
+    r.eta_issued = false;
+    NPPH_HIP(hipMemcpyAsync(fs->host, r.ctr, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    NPPH_HIP(hipEventRecord(fs->ev, s));
+    // eta stops every system after max_iter sweeps, so the counts reach zero
+    // by then; the bound only guards the loop
+    NPPH_TRY(corr_step(r, r.it, s));
+    NPPH_HIP(hipEventSynchronize(fs->ev));
+    if (fs->host[0] == 0 || r.it > max_iter_of(r.o)) break;
+    ++r.it;
+  }
+  *n_fb = fs->host[1];
+  return NPPH_OK;
+}
+
+int refine_end(RefineRun& r, int32_t n_fb, hipStream_t s) {
+  if (r.count <= 0 || r.n <= 0 || r.nrhs <= 0) return NPPH_OK;
+  if (r.o.precision != NPPH_PREC_FP64 && r.o.fallback_fp64 && n_fb > 0) {
+    hipLaunchKernelGGL(fallback_mask_kernel, g1(r.count), dim3(256), 0, s, r.count, r.active, r.mask, r.iters);
+    NPPH_TRY(fp64_solve(r, true, s));
+    NPPH_HIP(hipMemcpyAsync(r.active, r.mask, sizeof(int32_t) * r.count, hipMemcpyDeviceToDevice, s));
+    NPPH_TRY(report(r, s));
+  }
+  if (r.iters_out)
+    NPPH_HIP(hipMemcpyAsync(r.iters_out, r.iters, sizeof(int32_t) * r.count, hipMemcpyDeviceToDevice, s));
+  if (r.info_out) NPPH_HIP(hipMemcpyAsync(r.info_out, r.info, sizeof(int32_t) * r.count, hipMemcpyDeviceToDevice, s));
+  return NPPH_OK;
+}
+
+int refine_solve(const npph_refine_opts& o, const double* A, const RefineFactors& f, int nrhs, const double* B,
+                 double* X, double* berr, int32_t* iters_out, int32_t* info_out, hipStream_t s) {
+  RefineRun r;
+  r.o = o;
+  r.A = A;
+  r.f = f;
+  r.nrhs = nrhs;
+  r.B = B;
+  r.X = X;
+  r.berr_out = berr;
+  r.iters_out = iters_out;
+  r.info_out = info_out;
+  NPPH_TRY(refine_begin(r, s));
+  int32_t n_fb = 0;
+  NPPH_TRY(refine_loop(r, &n_fb, s));
+  return refine_end(r, n_fb, s);
 }
 
 int refined_impl(const npph_refine_opts& o, int64_t count, int n, int nrhs, const double* A, const double* B,

@@ -126,6 +126,7 @@ void ws_destroy(WsSet* w) {
 }
 
 void ws_bind(WsSet* w) { t_ws = w; }
+WsSet* ws_bound() { return t_ws; }
 bool ws_private() { return t_ws != nullptr && t_ws->priv; }
 
 void ws_release_stream(hipStream_t s) {

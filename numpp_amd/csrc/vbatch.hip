@@ -17,10 +17,20 @@
 // sum_{j<i} nrhs_j; all row-major.
 //
 // Every precision reports the normwise backward error of each result (FP64:
-// LU, getrs and one residual; FP32/BF16: refinement). The FP64 execution (pack,
-// per-bucket LU + solve + report, unpack) is captured once into a hipGraph and
-// replayed; the graph is re-captured when the data pointers or a workspace
-// address change.
+// LU, getrs and one residual; FP32/BF16: refinement).
+//
+// Buckets run concurrently: each is assigned to one of up to four branch
+// streams (largest estimated work first, onto the least-loaded branch) and
+// has its own workspace set, so buckets share no scratch and their launches
+// overlap on the device. The refinement of every bucket runs a fixed budget of
+// test + correction iterations with no host read-back, so the whole execution
+// (fork, per-bucket pack, LU, sweeps, unpack, join) is one stream-ordered
+// sequence that is captured once into a hipGraph and replayed. After it, one
+// read-back of each bucket's counts tells which buckets still have systems
+// iterating (or stopped, for the FP64 fallback); those alone continue eagerly
+// from the same device state, and the budget grows for later captures.
+#include <sys/stat.h>
+
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -44,7 +54,12 @@ int refined_impl(const npph_refine_opts& o, int64_t count, int n, int nrhs, cons
 namespace {
 
 struct Bucket {
-  int N = 0, R = 0;  // padded order, padded nrhs
+  int N = 0, R = 0;  // padded order, padded nrhs (the bucket's widest)
+  int br = 0;        // branch stream
+  double cost = 0.0; // estimated work (microseconds)
+  WsSet* ws = nullptr;  // the bucket's own workspace set
+  RefineRun run;     // the refined solve's state between capture and read-back
+  int it_after = 0;  // run.it after the budgeted sweeps
   std::vector<int32_t> ids;
   int32_t* d_ids = nullptr;
   double* A = nullptr;  // cnt x N x N
@@ -63,6 +78,14 @@ struct Plan {
   int64_t *d_aoff = nullptr, *d_boff = nullptr, *d_coff = nullptr;
   std::vector<Bucket> buckets;
   std::vector<void*> allocs;
+  std::vector<hipStream_t> branches;
+  std::vector<hipEvent_t> ev_join;
+  hipEvent_t ev_fork = nullptr;
+  hipEvent_t ev_chain = nullptr;  // per-launch profiling: buckets one after another
+  int32_t* d_rem = nullptr;  // per bucket: systems still iterating, stopped without convergence
+  int32_t* h_rem = nullptr;  // pinned host copy
+  int budget = 0;            // test + correction iterations issued without read-back
+  int gBudget = -1;
   hipGraphExec_t exec = nullptr;
   hipGraph_t graph = nullptr;
   const void* gA = nullptr;
@@ -78,11 +101,12 @@ struct Plan {
 
 // ---- pack / unpack ----
 
-// Packs one bucket and prepares its factorisation in the same pass: the padded
-// FP64 copy (kept for the residual), the working copy the LU overwrites (FP64,
-// or rounded to FP32 for the mixed-precision factorisation) and ||A||_inf of
-// the system itself (its n rows). One workgroup per system, one wave per row
-// (lanes over columns); then B's padded copy (zero rows and columns).
+// Packs one bucket and prepares its factorisation in the same pass: the working
+// copy the LU overwrites (FP64, or rounded to FP32 for the mixed-precision
+// factorisation), ||A||_inf of the system itself (its n rows) and, when Ap is
+// non-null (orders above 256), a padded FP64 copy for the residual. One
+// workgroup per system, one wave per row (lanes over columns); then B's padded
+// copy (zero rows and columns).
 template <class LO>
 __global__ __launch_bounds__(256) void pack_prep_kernel(int N, int R, const int32_t* ids, const int32_t* nn,
                                                         const int32_t* nr, const int64_t* aoff, const int64_t* boff,
@@ -93,7 +117,7 @@ __global__ __launch_bounds__(256) void pack_prep_kernel(int N, int R, const int3
   const int n = nn[id], m = nr[id];
   const double* __restrict__ a = A + aoff[id];
   const double* __restrict__ b = B + boff[id];
-  double* __restrict__ ap = Ap + k * (int64_t)N * N;
+  double* __restrict__ ap = Ap ? Ap + k * (int64_t)N * N : nullptr;
   LO* __restrict__ lo = Lo + k * (int64_t)N * N;
   double* __restrict__ bp = Bp + k * (int64_t)N * R;
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
@@ -107,7 +131,7 @@ __global__ __launch_bounds__(256) void pack_prep_kernel(int N, int R, const int3
       const bool in = i < n && j < n;
       const double v = in ? ar[min(j, n - 1)] : (i == j ? 1.0 : 0.0);
       s += in ? fabs(v) : 0.0;
-      ap[(int64_t)i * N + j] = v;
+      if (Ap) ap[(int64_t)i * N + j] = v;
       lo[(int64_t)i * N + j] = (LO)v;
     }
 #pragma unroll
@@ -177,14 +201,16 @@ void tune_load() {
   }
 }
 
+// mkdir -p of the cache file's directory (no shell)
+void make_dirs(const std::string& dir) {
+  for (size_t p = 1; p <= dir.size(); ++p)
+    if (p == dir.size() || dir[p] == '/') (void)::mkdir(dir.substr(0, p).c_str(), 0755);
+}
+
 void tune_save() {
   const std::string path = tune_path();
   const size_t slash = path.rfind('/');
-  if (slash != std::string::npos) {
-    std::string dir = path.substr(0, slash);
-    std::string cmd = "mkdir -p '" + dir + "' 2>/dev/null";
-    if (std::system(cmd.c_str()) != 0) return;
-  }
+  if (slash != std::string::npos && slash > 0) make_dirs(path.substr(0, slash));
   std::ofstream f(path);
   if (!f) return;
   f << "{";
@@ -259,7 +285,9 @@ int tune_cost(const npph_refine_opts& o, int N, int R, double* us, hipStream_t s
 }
 
 int round_up(int v, int g) { return (v + g - 1) / g * g; }
-int small_order(int n) { return n <= 8 ? 8 : n <= 16 ? 16 : n <= 32 ? 32 : 64; }
+// Orders up to 64 pad to a multiple of 8 (the small-order kernels take any
+// n <= 64; their cost follows the order they are given).
+int small_order(int n) { return round_up(n, 8); }
 
 // Bucket order for a system of order n under padding granularity G (n > 64).
 int bucket_order(int n, int G) { return n <= 64 ? small_order(n) : round_up(n, G); }
@@ -306,11 +334,18 @@ int plan_build(Plan* p, int autotune, hipStream_t s) {
   for (auto& kv : by) {
     Bucket b;
     b.N = kv.first;
-    b.R = R;
     b.ids = kv.second;
+    int rb = 1;
+    for (int32_t id : b.ids) rb = std::max(rb, (int)p->nrhs[id]);
+    b.R = rb;
     const size_t c = b.ids.size();
+    double us = 0.0;
+    if (autotune) NPPH_TRY(tune_cost(p->opts, b.N, R, &us, s));
+    b.cost = autotune ? us * (double)c : (double)c * b.N * b.N * (b.N + 64.0) * 1e-6;
     NPPH_TRY(dalloc(p, &b.d_ids, c));
-    NPPH_TRY(dalloc(p, &b.A, c * b.N * b.N));
+    // orders up to 256 keep their matrices packed (the residual reads them in
+    // place); larger ones get a padded FP64 copy
+    if (b.N > 256) NPPH_TRY(dalloc(p, &b.A, c * b.N * b.N));
     NPPH_TRY(dalloc(p, &b.B, c * b.N * b.R));
     NPPH_TRY(dalloc(p, &b.X, c * b.N * b.R));
     NPPH_TRY(dalloc(p, &b.berr, c * b.R));
@@ -319,35 +354,168 @@ int plan_build(Plan* p, int autotune, hipStream_t s) {
     NPPH_HIP(hipMemcpyAsync(b.d_ids, b.ids.data(), sizeof(int32_t) * c, hipMemcpyHostToDevice, s));
     p->buckets.push_back(std::move(b));
   }
+  // branches: up to four streams (the device's hardware queues per process),
+  // buckets placed largest first onto the least-loaded one
+  const int nbr = std::min<int>(4, (int)p->buckets.size());
+  std::vector<size_t> order(p->buckets.size());
+  for (size_t i = 0; i < order.size(); ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](size_t x, size_t y) { return p->buckets[x].cost > p->buckets[y].cost; });
+  std::vector<double> load(nbr, 0.0);
+  for (size_t i : order) {
+    const int br = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+    p->buckets[i].br = br;
+    load[br] += p->buckets[i].cost;
+  }
+  for (int i = 0; i < nbr; ++i) {
+    hipStream_t bs = nullptr;
+    NPPH_HIP(hipStreamCreateWithFlags(&bs, hipStreamNonBlocking));
+    p->branches.push_back(bs);
+    hipEvent_t e = nullptr;
+    NPPH_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    p->ev_join.push_back(e);
+  }
+  NPPH_HIP(hipEventCreateWithFlags(&p->ev_fork, hipEventDisableTiming));
+  NPPH_HIP(hipEventCreateWithFlags(&p->ev_chain, hipEventDisableTiming));
+  for (auto& b : p->buckets) b.ws = ws_create(p->branches[b.br]);
+  NPPH_TRY(dalloc(p, &p->d_rem, 2 * p->buckets.size()));
+  void* hp = nullptr;
+  NPPH_HIP(hipHostMalloc(&hp, sizeof(int32_t) * 2 * p->buckets.size(), hipHostMallocDefault));
+  p->h_rem = static_cast<int32_t*>(hp);
+  const bool bf16 = p->opts.precision == NPPH_PREC_BF16;
+  // the budget grows after an execution that needed more (finish_plan)
+  p->budget = p->opts.precision == NPPH_PREC_FP64 ? 0 : (bf16 ? 3 : 1);
   NPPH_HIP(hipStreamSynchronize(s));
   return NPPH_OK;
 }
 
-// One execution without graph capture.
+// Releases what plan_build made beyond the plain allocations.
+void plan_release(Plan* p) {
+  for (auto& b : p->buckets)
+    if (b.ws) ws_destroy(b.ws);
+  for (hipStream_t bs : p->branches) (void)hipStreamDestroy(bs);
+  for (hipEvent_t e : p->ev_join) (void)hipEventDestroy(e);
+  if (p->ev_fork) (void)hipEventDestroy(p->ev_fork);
+  if (p->ev_chain) (void)hipEventDestroy(p->ev_chain);
+  if (p->h_rem) (void)hipHostFree(p->h_rem);
+  for (void* q : p->allocs) (void)hipFree(q);
+}
+
+// Binds a bucket's workspace set for the scope, restoring the caller's.
+struct BucketScope {
+  WsSet* outer;
+  explicit BucketScope(WsSet* w) : outer(ws_bound()) { ws_bind(w); }
+  ~BucketScope() { ws_bind(outer); }
+};
+
+int unpack(Plan* p, Bucket& b, double* X, double* berr, int32_t* iters, int32_t* info, hipStream_t bs) {
+  hipLaunchKernelGGL(unpack_kernel, dim3((unsigned)b.ids.size()), dim3(256), 0, bs, b.N, b.R, b.d_ids, p->d_n,
+                     p->d_nrhs, p->d_boff, p->d_coff, b.X, b.info, b.berr, b.iters, X, info, berr, iters);
+  NPPH_HIP(hipGetLastError());
+  return NPPH_OK;
+}
+
+// One bucket's stream-ordered work on its branch: pack, factorise, solve
+// (FP64) or sweep within the budget (the other precisions), unpack.
+int issue_bucket(Plan* p, size_t bi, const double* A, const double* B, double* X, double* berr, int32_t* iters,
+                 int32_t* info, hipStream_t bs) {
+  Bucket& b = p->buckets[bi];
+  BucketScope scope(b.ws);
+  const int64_t c = (int64_t)b.ids.size();
+  void* buf;
+  NPPH_TRY(workspace(kWsLo, refine_factor_bytes(p->opts.precision, c, b.N), &buf));
+  RefineFactors f;
+  refine_factor_carve(p->opts.precision, c, b.N, buf, f);
+  if (p->opts.precision == NPPH_PREC_FP64)
+    hipLaunchKernelGGL(pack_prep_kernel<double>, dim3((unsigned)c), dim3(256), 0, bs, b.N, b.R, b.d_ids, p->d_n,
+                       p->d_nrhs, p->d_aoff, p->d_boff, A, B, b.A, b.B, f.lu64, f.anorm);
+  else
+    hipLaunchKernelGGL(pack_prep_kernel<float>, dim3((unsigned)c), dim3(256), 0, bs, b.N, b.R, b.d_ids, p->d_n,
+                       p->d_nrhs, p->d_aoff, p->d_boff, A, B, b.A, b.B, f.lu32, f.anorm);
+  NPPH_HIP(hipGetLastError());
+  NPPH_TRY(refine_factor(p->opts, b.A, f, bs, true));
+  RefineRun& r = b.run;
+  r = RefineRun();
+  r.o = p->opts;
+  r.A = b.A;
+  if (!b.A) r.var = PackedA{b.d_ids, p->d_n, p->d_aoff, A};
+  r.f = f;
+  r.nrhs = b.R;
+  r.B = b.B;
+  r.X = b.X;
+  r.berr_out = b.berr;
+  r.iters_out = b.iters;
+  r.info_out = b.info;
+  NPPH_TRY(refine_begin(r, bs));
+  if (p->opts.precision != NPPH_PREC_FP64) {
+    NPPH_TRY(refine_steps(r, p->budget, bs));
+    NPPH_HIP(hipMemcpyAsync(p->d_rem + 2 * bi, r.ctr, 2 * sizeof(int32_t), hipMemcpyDeviceToDevice, bs));
+  }
+  b.it_after = r.it;
+  NPPH_TRY(refine_end(r, 0, bs));
+  return unpack(p, b, X, berr, iters, info, bs);
+}
+
+// The stream-ordered part of an execution (what a graph captures): fork onto
+// the branches, every bucket, join back onto s.
 int run_plan(Plan* p, const double* A, const double* B, double* X, double* berr, int32_t* iters, int32_t* info,
              hipStream_t s) {
-  for (auto& b : p->buckets) {
-    const int64_t c = (int64_t)b.ids.size();
-    void* buf;
-    NPPH_TRY(workspace(kWsLo, refine_factor_bytes(p->opts.precision, c, b.N), &buf));
-    RefineFactors f;
-    refine_factor_carve(p->opts.precision, c, b.N, buf, f);
-    if (p->opts.precision == NPPH_PREC_FP64)
-      hipLaunchKernelGGL(pack_prep_kernel<double>, dim3((unsigned)c), dim3(256), 0, s, b.N, b.R, b.d_ids, p->d_n,
-                         p->d_nrhs, p->d_aoff, p->d_boff, A, B, b.A, b.B, f.lu64, f.anorm);
-    else
-      hipLaunchKernelGGL(pack_prep_kernel<float>, dim3((unsigned)c), dim3(256), 0, s, b.N, b.R, b.d_ids, p->d_n,
-                         p->d_nrhs, p->d_aoff, p->d_boff, A, B, b.A, b.B, f.lu32, f.anorm);
-    NPPH_HIP(hipGetLastError());
-    // FP64 runs LU + getrs + the backward-error report (iters = 0); the other
-    // precisions refine.
-    NPPH_TRY(refine_factor(p->opts, b.A, f, s, true));
-    NPPH_TRY(refine_solve(p->opts, b.A, f, b.R, b.B, b.X, b.berr, b.iters, b.info, s));
-    hipLaunchKernelGGL(unpack_kernel, dim3((unsigned)c), dim3(256), 0, s, b.N, b.R, b.d_ids, p->d_n, p->d_nrhs,
-                       p->d_boff, p->d_coff, b.X, b.info, b.berr, b.iters, X, info,
-                       berr, iters);
-    NPPH_HIP(hipGetLastError());
+  NPPH_HIP(hipEventRecord(p->ev_fork, s));
+  for (hipStream_t bs : p->branches) NPPH_HIP(hipStreamWaitEvent(bs, p->ev_fork, 0));
+  // Under per-launch profiling the buckets run one after another, so each
+  // launch's event interval holds that launch alone.
+  const bool serial = prof_enabled();
+  int st = NPPH_OK;
+  for (size_t i = 0; i < p->buckets.size() && st == NPPH_OK; ++i) {
+    hipStream_t bs = p->branches[p->buckets[i].br];
+    if (serial && i > 0) NPPH_HIP(hipStreamWaitEvent(bs, p->ev_chain, 0));
+    st = issue_bucket(p, i, A, B, X, berr, iters, info, bs);
+    if (serial) NPPH_HIP(hipEventRecord(p->ev_chain, bs));
   }
+  // join every branch even after a failure, so a capture can end cleanly
+  for (size_t k = 0; k < p->branches.size(); ++k) {
+    NPPH_HIP(hipEventRecord(p->ev_join[k], p->branches[k]));
+    NPPH_HIP(hipStreamWaitEvent(s, p->ev_join[k], 0));
+  }
+  return st;
+}
+
+// After run_plan (eager or replayed): buckets whose budget left systems
+// iterating continue with read-backs; buckets with systems stopped without
+// convergence get the FP64 fallback; both unpack again. The budget of later
+// captures grows to cover what was needed. Synchronises s (refining
+// precisions only).
+int finish_plan(Plan* p, double* X, double* berr, int32_t* iters, int32_t* info, hipStream_t s) {
+  if (p->opts.precision == NPPH_PREC_FP64) return NPPH_OK;
+  const size_t nb = p->buckets.size();
+  NPPH_HIP(hipMemcpyAsync(p->h_rem, p->d_rem, sizeof(int32_t) * 2 * nb, hipMemcpyDeviceToHost, s));
+  NPPH_HIP(hipStreamSynchronize(s));
+  std::vector<bool> used(p->branches.size(), false);
+  int need = p->budget;
+  for (size_t i = 0; i < nb; ++i) {
+    Bucket& b = p->buckets[i];
+    const int32_t act = p->h_rem[2 * i], fb = p->h_rem[2 * i + 1];
+    if (act <= 0 && !(fb > 0 && p->opts.fallback_fp64)) continue;
+    hipStream_t bs = p->branches[b.br];
+    BucketScope scope(b.ws);
+    b.run.it = b.it_after;  // the state the budgeted sweeps left (also after a replay)
+    b.run.eta_issued = true;
+    int32_t n_fb = fb;
+    if (act > 0) {
+      NPPH_TRY(refine_loop(b.run, &n_fb, bs));
+      need = std::max(need, b.run.it);  // the loop ended after the test of iteration `it`
+    }
+    NPPH_TRY(refine_end(b.run, n_fb, bs));
+    NPPH_TRY(unpack(p, b, X, berr, iters, info, bs));
+    used[b.br] = true;
+  }
+  for (size_t k = 0; k < p->branches.size(); ++k)
+    if (used[k]) {
+      NPPH_HIP(hipEventRecord(p->ev_join[k], p->branches[k]));
+      NPPH_HIP(hipStreamWaitEvent(s, p->ev_join[k], 0));
+    }
+  const int cap = (p->opts.max_iter > 0 ? p->opts.max_iter : 10) + 1;
+  p->budget = std::min(cap, need);
   return NPPH_OK;
 }
 
@@ -401,7 +569,8 @@ int npph_vbatch_plan_create(int64_t count, const int32_t* n, const int32_t* nrhs
   up(&p->d_coff, coff.data());
   if (st == NPPH_OK) st = plan_build(p, autotune, s);
   if (st != NPPH_OK) {
-    for (void* q : p->allocs) (void)hipFree(q);
+    (void)hipStreamSynchronize(s);
+    plan_release(p);
     delete p;
     return st;
   }
@@ -416,7 +585,7 @@ int npph_vbatch_plan_destroy(void* plan) {
   NPPH_HIP(hipDeviceSynchronize());
   if (p->exec) (void)hipGraphExecDestroy(p->exec);
   if (p->graph) (void)hipGraphDestroy(p->graph);
-  for (void* q : p->allocs) (void)hipFree(q);
+  plan_release(p);
   delete p;
   return NPPH_OK;
 }
@@ -442,18 +611,15 @@ int npph_vbatch_execute(void* plan, const double* A, const double* B, double* X,
   if (!p || !A || !B || !X) { set_error("vbatch: bad arguments"); return NPPH_ERR_INVALID; }
   hipStream_t s = stream ? (hipStream_t)stream : default_stream();
   CallScope cs(s);
-  const bool graphable = use_graph && p->opts.precision == NPPH_PREC_FP64;
-  if (!graphable) return run_plan(p, A, B, X, berr, iters, info, s);
-  // The eager warm-up and the capture issue the same launches on s.
-  PlainIssue plain;
-  // First call: run eagerly so every workspace slot reaches its final size.
-  if (!p->warmed) {
+  // First call: run eagerly so every bucket's workspace set reaches its final
+  // size; graphs are captured from the second call on.
+  if (!use_graph || !p->warmed) {
     NPPH_TRY(run_plan(p, A, B, X, berr, iters, info, s));
     p->warmed = true;
-    return NPPH_OK;
+    return finish_plan(p, X, berr, iters, info, s);
   }
-  const bool stale = !p->exec || p->gA != A || p->gB != B || p->gX != X || p->gI != info || p->gE != berr || p->gT != iters || p->gS != s ||
-                     p->gGen != workspace_generation();
+  bool stale = !p->exec || p->gA != A || p->gB != B || p->gX != X || p->gI != info || p->gE != berr ||
+               p->gT != iters || p->gS != s || p->gGen != workspace_generation() || p->gBudget != p->budget;
   if (stale) {
     if (p->exec) { (void)hipGraphExecDestroy(p->exec); p->exec = nullptr; }
     if (p->graph) { (void)hipGraphDestroy(p->graph); p->graph = nullptr; }
@@ -472,9 +638,10 @@ int npph_vbatch_execute(void* plan, const double* A, const double* B, double* X,
     p->graph = g;
     NPPH_HIP(hipGraphInstantiate(&p->exec, g, nullptr, nullptr, 0));
     p->gA = A; p->gB = B; p->gX = X; p->gI = info; p->gE = berr; p->gT = iters; p->gS = s; p->gGen = gen;
+    p->gBudget = p->budget;
   }
   NPPH_HIP(hipGraphLaunch(p->exec, s));
-  return NPPH_OK;
+  return finish_plan(p, X, berr, iters, info, s);
 }
 
 }  // extern "C"

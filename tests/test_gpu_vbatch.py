@@ -129,3 +129,67 @@ def test_graph_capture_fresh_process(hip):
     root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
     r = subprocess.run([sys.executable, "-c", FRESH, root], capture_output=True, text=True, timeout=180)
     assert r.returncode == 0, r.stdout + r.stderr
+
+
+def _refined_case(seed, count, conds):
+    rng = np.random.default_rng(seed)
+    mats, rhss = [], []
+    for i, n in enumerate(rng.integers(8, 257, count)):
+        mats.append(cond_matrix(rng, int(n), conds[i % len(conds)]))
+        rhss.append(rng.standard_normal((int(n), int(rng.integers(1, 17)))))
+    return mats, rhss
+
+
+@pytest.mark.parametrize("precision,conds", [("fp32", [1e2, 1e4]), ("fp32", [1e2, 1e5, 1e6]),
+                                             ("bf16", [1e3])])
+def test_refined_graph_replay_matches_eager(hip, precision, conds):
+    # Refinement runs a fixed budget of sweeps per bucket inside the graph and
+    # continues eagerly for buckets that still iterate (kappa 1e5..1e6 needs
+    # more sweeps than the first budget): every replay equals the eager run
+    # bitwise, iteration counts included, and meets the tolerance.
+    mats, rhss = _refined_case(11, 48, conds)
+    A, B, n, nr = vbatch.pack(mats, rhss)
+    plan = vbatch.VPlan(n, nr, precision)
+    dA, dB = DeviceArray.from_host(A), DeviceArray.from_host(B)
+    dX, dE = DeviceArray(B.shape), DeviceArray((int(nr.sum()),))
+    dT, dI = DeviceArray((len(n),), np.int32), DeviceArray((len(n),), np.int32)
+    plan.execute(dA, dB, dX, dE, dT, dI, use_graph=False)
+    eager, eiters, eberr = dX.to_host(), dT.to_host(), dE.to_host()
+    assert np.all(dI.to_host() == 0)
+    for rep in range(4):
+        dX.copy_from(np.zeros_like(B))
+        plan.execute(dA, dB, dX, dE, dT, dI, use_graph=True)
+        np.testing.assert_array_equal(dX.to_host(), eager)
+        np.testing.assert_array_equal(dT.to_host(), eiters)
+        np.testing.assert_array_equal(dE.to_host(), eberr)
+        plan.execute(dA, dB, dX, dE, dT, dI, use_graph=False)
+    plan.close()
+    off = 0
+    for a, b, x in zip(mats, rhss, vbatch.unpack(eager, n, nr)):
+        assert np.all(oracle.backward_error(a, b, x) <= 1e-12)
+        off += b.shape[1]
+    assert np.all(eberr <= 1e-12)
+
+
+def test_refined_fallback_in_buckets(hip):
+    # kappa 1e9 is beyond FP32 refinement: those systems are re-solved in FP64
+    # (iters = -1) while their bucket neighbours converge; graph and eager agree.
+    mats, rhss = _refined_case(12, 30, [1e2, 1e9, 1e3])
+    A, B, n, nr = vbatch.pack(mats, rhss)
+    plan = vbatch.VPlan(n, nr, "fp32")
+    dA, dB = DeviceArray.from_host(A), DeviceArray.from_host(B)
+    dX, dE = DeviceArray(B.shape), DeviceArray((int(nr.sum()),))
+    dT, dI = DeviceArray((len(n),), np.int32), DeviceArray((len(n),), np.int32)
+    outs = []
+    for g in (False, True, True, True):
+        dX.copy_from(np.zeros_like(B))
+        plan.execute(dA, dB, dX, dE, dT, dI, use_graph=g)
+        outs.append((dX.to_host(), dT.to_host()))
+    plan.close()
+    for x, t in outs[1:]:
+        np.testing.assert_array_equal(x, outs[0][0])
+        np.testing.assert_array_equal(t, outs[0][1])
+    iters = outs[0][1]
+    assert np.all(iters[1::3] == -1) and np.all(iters[0::3] > 0)
+    for a, b, x in zip(mats, rhss, vbatch.unpack(outs[0][0], n, nr)):
+        assert np.all(oracle.backward_error(a, b, x) <= 1e-12)
