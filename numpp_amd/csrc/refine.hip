@@ -376,7 +376,6 @@ __global__ __launch_bounds__(256) void residual_var_kernel(int N, int nrhs, cons
                                                            const int64_t* aoff, const double* Apk, const double* Xb,
                                                            const double* Bb, double* Rb, double* rmax, double* xmax,
                                                            const int32_t* active) {
-  constexpr int KU = 4;
   const int64_t mat = blockIdx.x;
   if (active && active[mat] != 1) return;
   const int id = ids[mat], n = nn[id];
@@ -401,41 +400,50 @@ __global__ __launch_bounds__(256) void residual_var_kernel(int N, int nrhs, cons
   xm = fmax(xm, __shfl_xor(xm, 32, 64));
   if (lane < 16) red_x[wave][lane] = xm;
   __syncthreads();
-  // 16-byte loads need an even pitch and a 16-byte aligned start
-  const bool vec = (n % 2 == 0) && n >= 2 && ((reinterpret_cast<uintptr_t>(A) & 15) == 0);
+  // pairs of doubles per load: a packed matrix starts on 8 bytes only, so
+  // the pair type is 8-byte aligned (the device supports such vector loads)
+  typedef double d2u __attribute__((ext_vector_type(2), aligned(8)));
+  constexpr int KV = 8;  // 16-column chunks in flight per step
   double rm = 0.0;
   for (int r0 = wave * 16; r0 < N; r0 += 64) {
     v4d acc = {0.0, 0.0, 0.0, 0.0};
     if (r0 < n) {
+      // rows past n read row n - 1: output row i depends on operand row i
+      // only, and rows i >= n take the identity below
       const int row = min(r0 + lr, n - 1);
-      const bool live = r0 + lr < n;
       const double* __restrict__ ar = A + (int64_t)row * n;
-      for (int k0 = 0; k0 < n; k0 += 16 * KU) {
-        double a[KU][4];
+      for (int k0 = 0; k0 < n; k0 += 16 * KV) {
+        // pairs (k, k + 1) and (k + 2, k + 3), each clamped to start at most
+        // at n - 2 so it stays inside the row (n >= 2; n = 1 clamps to 0 and
+        // reads the row's single entry twice); an index past the clamp
+        // (k = n - 1 for odd n) takes the pair's second entry, and entries
+        // past the row are zero in the operand
+        d2u pr[KV][2];
 #pragma unroll
-        for (int u = 0; u < KU; ++u) {
+        for (int u = 0; u < KV; ++u) {
           const int k = k0 + 16 * u + 4 * lk;
-          if (vec) {
-            const double2 p0 = *reinterpret_cast<const double2*>(ar + min(k, n - 2));
-            const double2 p1 = *reinterpret_cast<const double2*>(ar + min(k + 2, n - 2));
-            a[u][0] = k < n ? p0.x : 0.0;
-            a[u][1] = k + 1 < n ? p0.y : 0.0;
-            a[u][2] = k + 2 < n ? p1.x : 0.0;
-            a[u][3] = k + 3 < n ? p1.y : 0.0;
+          if (n >= 2) {
+            pr[u][0] = *reinterpret_cast<const d2u*>(ar + min(k, n - 2));
+            pr[u][1] = *reinterpret_cast<const d2u*>(ar + min(k + 2, n - 2));
           } else {
-#pragma unroll
-            for (int q = 0; q < 4; ++q) a[u][q] = k + q < n ? ar[min(k + q, n - 1)] : 0.0;
+            pr[u][0] = d2u{ar[0], 0.0};
+            pr[u][1] = d2u{0.0, 0.0};
           }
-#pragma unroll
-          for (int q = 0; q < 4; ++q) a[u][q] = live ? a[u][q] : 0.0;
         }
 #pragma unroll
-        for (int u = 0; u < KU; ++u) {
+        for (int u = 0; u < KV; ++u) {
+          const int k = k0 + 16 * u + 4 * lk;
+          const int b0 = min(k, n - 2), b1 = min(k + 2, n - 2);
+          const double a0 = k < n ? (k == b0 || n < 2 ? pr[u][0].x : pr[u][0].y) : 0.0;
+          const double a1 = k + 1 < n ? pr[u][0].y : 0.0;
+          const double a2 = k + 2 < n ? (k + 2 == b1 ? pr[u][1].x : pr[u][1].y) : 0.0;
+          const double a3 = k + 3 < n ? pr[u][1].y : 0.0;
           const int kb = k0 + 16 * u + 4 * lk;
+          const double av[4] = {a0, a1, a2, a3};
 #pragma unroll
           for (int q = 0; q < 4; ++q) {
             const int kx = kb + q < np ? kb + q : 0;
-            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u][q], xs[kx][lr], acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[q], xs[kx][lr], acc, 0, 0, 0);
           }
         }
       }

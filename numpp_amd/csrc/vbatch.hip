@@ -104,15 +104,19 @@ struct Plan {
 // Packs one bucket and prepares its factorisation in the same pass: the working
 // copy the LU overwrites (FP64, or rounded to FP32 for the mixed-precision
 // factorisation), ||A||_inf of the system itself (its n rows) and, when Ap is
-// non-null (orders above 256), a padded FP64 copy for the residual. One
-// workgroup per system, one wave per row (lanes over columns); then B's padded
-// copy (zero rows and columns).
+// non-null (orders above 256), a padded FP64 copy for the residual. A
+// workgroup takes 16 rows of one system (grid: systems x ceil(N / 16)), a wave
+// four of them (lanes over columns), and the same rows of B's padded copy
+// (zero rows and columns); row sums meet in anorm by atomic maximum (anorm
+// starts at zero).
+constexpr int PACK_ROWS = 16;
 template <class LO>
 __global__ __launch_bounds__(256) void pack_prep_kernel(int N, int R, const int32_t* ids, const int32_t* nn,
                                                         const int32_t* nr, const int64_t* aoff, const int64_t* boff,
                                                         const double* A, const double* B, double* Ap, double* Bp,
                                                         LO* Lo, double* anorm) {
   const int64_t k = blockIdx.x;
+  const int i0 = blockIdx.y * PACK_ROWS;
   const int id = ids[k];
   const int n = nn[id], m = nr[id];
   const double* __restrict__ a = A + aoff[id];
@@ -121,9 +125,11 @@ __global__ __launch_bounds__(256) void pack_prep_kernel(int N, int R, const int3
   LO* __restrict__ lo = Lo + k * (int64_t)N * N;
   double* __restrict__ bp = Bp + k * (int64_t)N * R;
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-  __shared__ double red[4];
   double amax = 0.0;
-  for (int i = wave; i < N; i += 4) {
+#pragma unroll
+  for (int rr = 0; rr < PACK_ROWS / 4; ++rr) {
+    const int i = i0 + 4 * wave + rr;
+    if (i >= N) break;  // uniform over the wave
     const double* __restrict__ ar = a + (int64_t)min(i, n - 1) * n;
     double s = 0.0;
 #pragma unroll 4
@@ -131,20 +137,25 @@ __global__ __launch_bounds__(256) void pack_prep_kernel(int N, int R, const int3
       const bool in = i < n && j < n;
       const double v = in ? ar[min(j, n - 1)] : (i == j ? 1.0 : 0.0);
       s += in ? fabs(v) : 0.0;
-      if (Ap) ap[(int64_t)i * N + j] = v;
+      if (ap) ap[(int64_t)i * N + j] = v;
       lo[(int64_t)i * N + j] = (LO)v;
     }
 #pragma unroll
     for (int q = 32; q >= 1; q >>= 1) s += __shfl_xor(s, q, 64);
     amax = fmax(amax, s);
   }
-  if (lane == 0) red[wave] = amax;
-  for (int e = t; e < N * R; e += 256) {
-    const int i = e / R, c = e - i * R;
-    bp[e] = (i < n && c < m) ? b[(int64_t)i * m + c] : 0.0;
+  if (lane == 0 && i0 + 4 * wave < n)
+    atomicMax(reinterpret_cast<unsigned long long*>(anorm + k), (unsigned long long)__double_as_longlong(amax));
+  const int rows = min(PACK_ROWS, N - i0);
+  for (int e = t; e < rows * R; e += 256) {
+    const int i = i0 + e / R, c = e % R;
+    bp[(int64_t)i * R + c] = (i < n && c < m) ? b[(int64_t)i * m + c] : 0.0;
   }
-  __syncthreads();
-  if (t == 0) anorm[k] = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+}
+
+__global__ void zero_f64_kernel(double* p, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0.0;
 }
 
 __global__ __launch_bounds__(256) void unpack_kernel(int N, int R, const int32_t* ids, const int32_t* nn,
@@ -331,17 +342,44 @@ int plan_build(Plan* p, int autotune, hipStream_t s) {
   }
   std::map<int, std::vector<int32_t>> by;
   for (int64_t i = 0; i < count; ++i) by[bucket_order(p->n[i], bestG)].push_back((int32_t)i);
+  // Buckets with their estimated work; a bucket holding more than a quarter
+  // of the whole is cut into equal parts (each at least 256 systems), so the
+  // branches can share the largest orders. Kernels work per system, so where a
+  // system lands does not change its result.
+  struct Part {
+    int N;
+    std::vector<int32_t> ids;
+    double cost;
+  };
+  std::vector<Part> parts;
+  double total = 0.0;
   for (auto& kv : by) {
+    double us = 0.0;
+    if (autotune) NPPH_TRY(tune_cost(p->opts, kv.first, R, &us, s));
+    const double per = autotune ? us : (double)kv.first * kv.first * (kv.first + 64.0) * 1e-6;
+    parts.push_back({kv.first, kv.second, per * (double)kv.second.size()});
+    total += parts.back().cost;
+  }
+  std::vector<Part> cut;
+  for (auto& q : parts) {
+    const int64_t c = (int64_t)q.ids.size();
+    int pieces = (int)std::min<int64_t>((int64_t)std::ceil(q.cost / (0.25 * total)), c / 256);
+    pieces = std::max(1, pieces);
+    for (int k = 0; k < pieces; ++k) {
+      const int64_t lo = c * k / pieces, hi = c * (k + 1) / pieces;
+      cut.push_back({q.N, std::vector<int32_t>(q.ids.begin() + lo, q.ids.begin() + hi),
+                     q.cost * (double)(hi - lo) / (double)c});
+    }
+  }
+  for (auto& q : cut) {
     Bucket b;
-    b.N = kv.first;
-    b.ids = kv.second;
+    b.N = q.N;
+    b.ids = std::move(q.ids);
     int rb = 1;
     for (int32_t id : b.ids) rb = std::max(rb, (int)p->nrhs[id]);
     b.R = rb;
     const size_t c = b.ids.size();
-    double us = 0.0;
-    if (autotune) NPPH_TRY(tune_cost(p->opts, b.N, R, &us, s));
-    b.cost = autotune ? us * (double)c : (double)c * b.N * b.N * (b.N + 64.0) * 1e-6;
+    b.cost = q.cost;
     NPPH_TRY(dalloc(p, &b.d_ids, c));
     // orders up to 256 keep their matrices packed (the residual reads them in
     // place); larger ones get a padded FP64 copy
@@ -426,11 +464,13 @@ int issue_bucket(Plan* p, size_t bi, const double* A, const double* B, double* X
   NPPH_TRY(workspace(kWsLo, refine_factor_bytes(p->opts.precision, c, b.N), &buf));
   RefineFactors f;
   refine_factor_carve(p->opts.precision, c, b.N, buf, f);
+  hipLaunchKernelGGL(zero_f64_kernel, dim3((unsigned)ceil_div(c, 256)), dim3(256), 0, bs, f.anorm, c);
+  const dim3 pgrid((unsigned)c, (unsigned)ceil_div(b.N, PACK_ROWS));
   if (p->opts.precision == NPPH_PREC_FP64)
-    hipLaunchKernelGGL(pack_prep_kernel<double>, dim3((unsigned)c), dim3(256), 0, bs, b.N, b.R, b.d_ids, p->d_n,
+    hipLaunchKernelGGL(pack_prep_kernel<double>, pgrid, dim3(256), 0, bs, b.N, b.R, b.d_ids, p->d_n,
                        p->d_nrhs, p->d_aoff, p->d_boff, A, B, b.A, b.B, f.lu64, f.anorm);
   else
-    hipLaunchKernelGGL(pack_prep_kernel<float>, dim3((unsigned)c), dim3(256), 0, bs, b.N, b.R, b.d_ids, p->d_n,
+    hipLaunchKernelGGL(pack_prep_kernel<float>, pgrid, dim3(256), 0, bs, b.N, b.R, b.d_ids, p->d_n,
                        p->d_nrhs, p->d_aoff, p->d_boff, A, B, b.A, b.B, f.lu32, f.anorm);
   NPPH_HIP(hipGetLastError());
   NPPH_TRY(refine_factor(p->opts, b.A, f, bs, true));
