@@ -413,6 +413,15 @@ int launch_sweep(int cw, dim3 grid, size_t shm, hipStream_t s, int n, int nrhs, 
 #undef NPPH_SWEEP
 }
 
+// NUMPP_HIP_GETRS_MW=0 keeps the single-workgroup sweeps for every shape.
+bool getrs_mw_off() {
+  static const bool off = [] {
+    const char* e = std::getenv("NUMPP_HIP_GETRS_MW");
+    return e && e[0] == '0';
+  }();
+  return off;
+}
+
 template <class T, bool REF>
 int getrs_impl(int64_t count, int n, int nrhs, const T* LU, int64_t strideA, const int32_t* perm, const T* B,
                int64_t strideB, T* X, int64_t strideX, const int32_t* info, const int32_t* mask, const RefIO& ref,
@@ -420,6 +429,12 @@ int getrs_impl(int64_t count, int n, int nrhs, const T* LU, int64_t strideA, con
   if (count <= 0 || n <= 0 || nrhs <= 0) return NPPH_OK;
   if (!REF && B == X) { set_error("getrs: B and X must not alias"); return NPPH_ERR_INVALID; }
   if (count > 2147483647LL) { set_error("getrs: count too large"); return NPPH_ERR_INVALID; }
+  if (getrs_mw_wanted(count, n, nrhs) && !getrs_mw_off()) {
+    ProfScope ps(kProfGetrs, 2.0 * count * n * n * nrhs,
+                 (double)count * (sizeof(T) * (double)n * n + 3.0 * sizeof(T) * n * nrhs), s);
+    return getrs_mw<T, REF>(count, n, nrhs, LU, strideA, perm, B, strideB, X, strideX, info, mask, ref.R, ref.scale,
+                            ref.X, ref.first, s);
+  }
   const int cw = chunk_width<T>(count, n, nrhs);
   const int nz = (int)ceil_div(nrhs, cw);
   if (nz > 65535) { set_error("getrs: nrhs too large"); return NPPH_ERR_INVALID; }

@@ -71,6 +71,17 @@ int getrs_refine(int64_t count, int n, int nrhs, const float* LU, int64_t stride
                  const double* R, const double* scale, float* Y, double* X, int first, const int32_t* mask,
                  hipStream_t s);
 
+// Sweeps with many workgroups per system (getrs_mw.hip): wanted for stacks of
+// few systems of order >= 512, where one workgroup per system and column chunk
+// would leave most CUs idle. Same contract as blocked_getrs_perm (REF: the
+// refinement correction of getrs_refine, with R, scale and the FP64 iterate
+// X64; X then holds the FP32 forward result and correction).
+bool getrs_mw_wanted(int64_t count, int n, int nrhs);
+template <class T, bool REF>
+int getrs_mw(int64_t count, int n, int nrhs, const T* LU, int64_t strideA, const int32_t* perm, const T* B,
+             int64_t strideB, T* X, int64_t strideX, const int32_t* info, const int32_t* mask, const double* R,
+             const double* scale, double* X64, int first, hipStream_t s);
+
 // perm from LAPACK-style pivots (replays the swaps).
 int ipiv_to_perm(int64_t count, int n, const int32_t* ipiv, int32_t* perm, hipStream_t s);
 
