@@ -44,7 +44,7 @@ struct Workspace {
 int ensure_init();                // NPPH_OK or NPPH_ERR_NO_DEVICE
 hipStream_t default_stream();     // library-owned stream for host-pointer calls
 int cu_count();                   // compute units on the device
-enum WsSlot { kWsA = 0, kWsB, kWsPiv, kWsInfo, kWsLo, kWsR, kWsX, kWsNorm, kWsMisc, kWsMove, kWsPerm, kWsD, kWsNet, kWsFlags, kWsPart, kWsSync, kWsCount };
+enum WsSlot { kWsA = 0, kWsB, kWsPiv, kWsInfo, kWsLo, kWsR, kWsX, kWsNorm, kWsMisc, kWsMove, kWsPerm, kWsD, kWsNet, kWsFlags, kWsPart, kWsSync, kWsSplitA, kWsSplitB, kWsCount };
 
 // Scratch memory. Every entry point runs inside a CallScope for the stream it
 // issues on. The scope binds that stream's workspace set to the calling thread

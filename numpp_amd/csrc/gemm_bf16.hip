@@ -3,19 +3,25 @@
 // (v_mfma_f32_16x16x32_bf16: A[i][k] with i = lane%16, k = 8*(lane/16)+0..7;
 // B[k][j] with j = lane%16, same k; four FP32 outputs C[4*(lane/16)+v][lane%16]).
 //
-// Each FP32 operand x is first scaled by a power of two chosen per workgroup
-// tile panel (the 128 x 64 block of L21 and the 64 x 128 block of U12 it reads),
-// so the largest magnitude in the panel lands in [1, 2); then split into BF16
-// pieces x = h + m + l (h = bf16(x), m = bf16(x - h), l = bf16(x - h - m)).
-// terms = 1 keeps only h (plain BF16); 3 uses h,l-two-piece products hh, hl,
-// lh (about 16 significant bits); 6 uses three pieces and the products hh, hm,
-// mh, hl, mm, lh (about FP32 accuracy). Products are accumulated smallest
-// first into one FP32 accumulator, then unscaled in the epilogue.
+// Two phases per update:
+//  1. split: each operand is scaled by a power of two per panel (a 64-row block
+//     of A spanning all of K, a 64-column block of B spanning all of K) so the
+//     panel's largest magnitude lands in [1, 2), then split into BF16 planes
+//     x = h + m + l (h = bf16(x), m = bf16(x - h), l = bf16(x - h - m)) and
+//     stored K-contiguous (A row-major [row][k], B transposed [col][k]), K padded
+//     with zeros to a multiple of 32 and rows/columns to a multiple of 128;
+//  2. product: a plain BF16 MFMA GEMM over the planes, 128 x 128 tiles, 32-deep
+//     K slices double-buffered through LDS, the partial products of one slice
+//     accumulated smallest first into one FP32 accumulator per output; the
+//     epilogue unscales by 2^(eA + eB) of the output's panels and subtracts.
+// terms = 1 keeps only h (plain BF16); 3 uses two pieces (h, m) and the
+// products hh, hm, mh (about 16 significant bits); 6 uses three pieces and hh,
+// hm, mh, hl, mm, lh (about FP32 accuracy).
 //
-// Workgroup: 256 threads (2 x 2 waves), tile 128 x 128. K is processed in
-// chunks of 64 (zero padded to a multiple of 32); each chunk gets its own
-// power-of-two scales and its partial products are unscaled into the FP32
-// accumulator.
+// The operands are split once per update instead of once per output tile, so
+// the product phase is a pure MFMA stream.
+#include <algorithm>
+
 #include "common.hpp"
 #include "gemm.hpp"
 #include "prof.hpp"
@@ -24,30 +30,177 @@ namespace npph {
 
 namespace {
 
-constexpr int TM = 128, TN = 128, KMAX = 64, KP = KMAX + 8;  // KP: padded LDS row
+constexpr int SPAN = 64;       // panel: rows of A / columns of B sharing one scale
+constexpr int TM = 128, TN = 128, BK = 32, KPAD = BK + 8;  // KPAD: LDS row (bf16)
 constexpr int NTH = 256;
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
+// Power-of-two exponent putting the panel's largest magnitude in [1, 2); a
+// zero, Inf or NaN panel is not scaled (Inf and NaN then reach C as they are),
+// and exponents are clamped so that the scale and its inverse stay finite.
+__device__ __forceinline__ int panel_exp(float amax) {
+  return (amax > 0.f && isfinite(amax)) ? min(max(ilogbf(amax), -126), 126) : 0;
+}
+
+__device__ __forceinline__ float block_max(float v, float* red) {
+#pragma unroll
+  for (int msk = 1; msk < 64; msk <<= 1) v = fmaxf(v, __shfl_xor(v, msk, 64));
+  const int t = threadIdx.x;
+  if ((t & 63) == 0) red[t >> 6] = v;
+  __syncthreads();
+  v = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+  return v;
+}
+
+// Splits x into NP BF16 pieces.
 template <int NP>
-__global__ __launch_bounds__(NTH) void bf16split_kernel(int64_t M, int64_t N, int K, const float* Ab,
-                                                         int64_t lda, int64_t sA, const float* Bb,
-                                                         int64_t ldb, int64_t sB, float* Cb, int64_t ldc,
-                                                         int64_t sC, int tiles_n) {
-  __shared__ __bf16 As[NP][TM][KP];
-  __shared__ __bf16 Bs[NP][TN][KP];
-  __shared__ float red[2][NTH / 64];
+__device__ __forceinline__ void split_val(float x, __bf16 (&o)[3]) {
+  const __bf16 h = (__bf16)x;
+  o[0] = h;
+  if constexpr (NP >= 2) {
+    const float r1 = x - (float)h;
+    const __bf16 m = (__bf16)r1;
+    o[1] = m;
+    if constexpr (NP >= 3) o[2] = (__bf16)(r1 - (float)m);
+  }
+}
+
+// A (M x K, row stride lda, matrix stride sA): one workgroup per 64-row panel.
+// Planes: P[((bz * NP + p) * Mp + row) * Kp + k]; exponents eA[bz * (Mp/64) + panel].
+template <int NP>
+__global__ __launch_bounds__(NTH) void split_rows_kernel(int64_t M, int K, int Kp, const float* __restrict__ Ab,
+                                                         int64_t lda, int64_t sA, __bf16* __restrict__ P, int64_t Mp,
+                                                         int* __restrict__ eA) {
+  __shared__ float red[4];
+  const int64_t bz = blockIdx.y;
+  const int panel = blockIdx.x;
+  const float* A = Ab + bz * sA;
+  const int t = threadIdx.x;
+  const int r = t >> 2, q = t & 3;  // 4 threads per row
+  const int64_t row = (int64_t)panel * SPAN + r;
+  const bool live = row < M;
+  const float* ar = A + (live ? row : 0) * lda;
+  float amax = 0.f;
+  if (live)
+    for (int k = q; k < K; k += 4) amax = fmaxf(amax, fabsf(ar[k]));
+  amax = block_max(amax, red);
+  const int e = panel_exp(amax);
+  if (t == 0) eA[bz * (Mp / SPAN) + panel] = e;
+  const float sc = ldexpf(1.f, -e);
+  // thread q writes k = 8 * (q + 4 j) .. + 7 as 16-byte vectors
+  for (int k0 = 8 * q; k0 < Kp; k0 += 32) {
+    bf16x8 v[3];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int k = k0 + u;
+      const float a = ar[min(k, K - 1)];  // clamped: padding reads a real entry
+      const float x = (live && k < K) ? a * sc : 0.f;
+      __bf16 o[3];
+      split_val<NP>(x, o);
+#pragma unroll
+      for (int p = 0; p < NP; ++p) v[p][u] = o[p];
+    }
+#pragma unroll
+    for (int p = 0; p < NP; ++p)
+      *reinterpret_cast<bf16x8*>(P + ((bz * NP + p) * Mp + row) * Kp + k0) = v[p];
+  }
+}
+
+// B (K x N, row stride ldb, matrix stride sB): one workgroup per 64-column
+// panel, transposed through LDS in 64 x 64 chunks.
+// Planes: P[((bz * NP + p) * Np + col) * Kp + k]; exponents eB[bz * (Np/64) + panel].
+template <int NP>
+__global__ __launch_bounds__(NTH) void split_cols_kernel(int64_t N, int K, int Kp, const float* __restrict__ Bb,
+                                                         int64_t ldb, int64_t sB, __bf16* __restrict__ P, int64_t Np,
+                                                         int* __restrict__ eB) {
+  __shared__ float red[4];
+  __shared__ float tile[SPAN][SPAN + 1];  // [k][col]
+  const int64_t bz = blockIdx.y;
+  const int panel = blockIdx.x;
+  const float* B = Bb + bz * sB;
+  const int t = threadIdx.x;
+  const int c = t & 63, kr = t >> 6;  // read: column c, rows kr, kr + 4, ...
+  const int64_t col = (int64_t)panel * SPAN + c;
+  const bool live = col < N;
+  float amax = 0.f;
+  if (live)
+    for (int k = kr; k < K; k += 4) amax = fmaxf(amax, fabsf(B[(int64_t)k * ldb + col]));
+  amax = block_max(amax, red);
+  const int e = panel_exp(amax);
+  if (t == 0) eB[bz * (Np / SPAN) + panel] = e;
+  const float sc = ldexpf(1.f, -e);
+  // write: column wc, k = kc0 .. kc0 + 15 of the chunk (two 16-byte vectors per plane)
+  const int wc = t >> 2, kq = (t & 3) * 16;
+  for (int kb = 0; kb < Kp; kb += SPAN) {
+    __syncthreads();  // previous chunk's tile reads are done
+#pragma unroll
+    for (int u = 0; u < SPAN / 4; ++u) {
+      const int k = kb + kr + 4 * u;
+      const float b = B[(int64_t)min(k, K - 1) * ldb + (live ? col : 0)];  // clamped
+      tile[kr + 4 * u][c] = (live && k < K) ? b * sc : 0.f;
+    }
+    __syncthreads();
+    const int64_t wcol = (int64_t)panel * SPAN + wc;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int k0 = kb + kq + 8 * h;
+      if (k0 >= Kp) continue;
+      bf16x8 v[3];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        __bf16 o[3];
+        split_val<NP>(tile[kq + 8 * h + u][wc], o);
+#pragma unroll
+        for (int p = 0; p < NP; ++p) v[p][u] = o[p];
+      }
+#pragma unroll
+      for (int p = 0; p < NP; ++p)
+        *reinterpret_cast<bf16x8*>(P + ((bz * NP + p) * Np + wcol) * Kp + k0) = v[p];
+    }
+  }
+}
+
+// C -= 2^(eA + eB) * sum over products of the planes. Tile 128 x 128, four waves
+// in a 2 x 2 grid of 64 x 64 sub-tiles (4 x 4 MFMA tiles each).
+template <int NP>
+__global__ __launch_bounds__(NTH) void bf16p_gemm_kernel(int64_t M, int64_t N, int Kp, const __bf16* __restrict__ PA,
+                                                         int64_t Mp, const int* __restrict__ eA,
+                                                         const __bf16* __restrict__ PB, int64_t Np,
+                                                         const int* __restrict__ eB, float* __restrict__ Cb,
+                                                         int64_t ldc, int64_t sC, int tiles_n) {
+  __shared__ __bf16 As[2][NP][TM][KPAD];
+  __shared__ __bf16 Bs[2][NP][TN][KPAD];
   const int64_t bz = blockIdx.y;
   const int tm = blockIdx.x / tiles_n, tn = blockIdx.x - tm * tiles_n;
-  const float* __restrict__ A = Ab + bz * sA;
-  const float* __restrict__ B = Bb + bz * sB;
-  float* __restrict__ C = Cb + bz * sC;
   const int64_t m0 = (int64_t)tm * TM, n0 = (int64_t)tn * TN;
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int wm = wave >> 1, wn = wave & 1;
   const int li = lane & 15, lk = (lane >> 4) * 8;
-  // Staging: thread t owns A row (t % 128) and B column (t % 128) of the
-  // chunk, k range [32 * (t / 128), +32).
-  const int r = t & 127, kh = (t >> 7) * 32;
-  const int64_t gr = m0 + r, gc = n0 + r;
+  // staging: thread t copies row (t >> 1), k half (t & 1) * 16 of each plane
+  const int sr = t >> 1, sk = (t & 1) * 16;
+  const __bf16* pa = PA + (bz * NP * Mp + m0 + sr) * Kp + sk;
+  const __bf16* pb = PB + (bz * NP * Np + n0 + sr) * Kp + sk;
+  const int64_t plA = Mp * Kp, plB = Np * Kp;
+
+  bf16x8 ra[NP][2], rb[NP][2];
+  auto load = [&](int k0) {
+#pragma unroll
+    for (int p = 0; p < NP; ++p)
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        ra[p][h] = *reinterpret_cast<const bf16x8*>(pa + p * plA + k0 + 8 * h);
+        rb[p][h] = *reinterpret_cast<const bf16x8*>(pb + p * plB + k0 + 8 * h);
+      }
+  };
+  auto store = [&](int buf) {
+#pragma unroll
+    for (int p = 0; p < NP; ++p)
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        *reinterpret_cast<bf16x8*>(&As[buf][p][sr][sk + 8 * h]) = ra[p][h];
+        *reinterpret_cast<bf16x8*>(&Bs[buf][p][sr][sk + 8 * h]) = rb[p][h];
+      }
+  };
 
   v4f acc[4][4];
 #pragma unroll
@@ -55,114 +208,91 @@ __global__ __launch_bounds__(NTH) void bf16split_kernel(int64_t M, int64_t N, in
 #pragma unroll
     for (int j = 0; j < 4; ++j) acc[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
 
-  for (int kc = 0; kc < K; kc += KMAX) {
-    const int kn = min(KMAX, K - kc);
-    float av[32], bv[32];
-    float amax = 0.f, bmax = 0.f;
+  const int nk = Kp / BK;
+  load(0);
+  store(0);
+  __syncthreads();
+  for (int kt = 0; kt < nk; ++kt) {
+    const int cur = kt & 1;
+    if (kt + 1 < nk) load((kt + 1) * BK);
+    bf16x8 a[NP][4], b[NP][4];
 #pragma unroll
-    for (int q = 0; q < 32; ++q) {
-      const int k = kh + q;
-      av[q] = (gr < M && k < kn) ? A[gr * lda + kc + k] : 0.f;
-      bv[q] = (gc < N && k < kn) ? B[(int64_t)(kc + k) * ldb + gc] : 0.f;
-      amax = fmaxf(amax, fabsf(av[q]));
-      bmax = fmaxf(bmax, fabsf(bv[q]));
+    for (int p = 0; p < NP; ++p) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) a[p][i] = *reinterpret_cast<const bf16x8*>(&As[cur][p][wm * 64 + i * 16 + li][lk]);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) b[p][j] = *reinterpret_cast<const bf16x8*>(&Bs[cur][p][wn * 64 + j * 16 + li][lk]);
     }
 #pragma unroll
-    for (int msk = 1; msk < 64; msk <<= 1) {
-      amax = fmaxf(amax, __shfl_xor(amax, msk, 64));
-      bmax = fmaxf(bmax, __shfl_xor(bmax, msk, 64));
+    for (int i = 0; i < 4; ++i) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        v4f c = acc[i][j];
+        if constexpr (NP == 3) {
+          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2][i], b[0][j], c, 0, 0, 0);
+          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1][i], b[1][j], c, 0, 0, 0);
+          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][i], b[2][j], c, 0, 0, 0);
+        }
+        if constexpr (NP >= 2) {
+          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1][i], b[0][j], c, 0, 0, 0);
+          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][i], b[1][j], c, 0, 0, 0);
+        }
+        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][i], b[0][j], c, 0, 0, 0);
+        acc[i][j] = c;
+      }
     }
-    if (kc > 0) __syncthreads();  // previous chunk's LDS reads are done
-    if (lane == 0) { red[0][wave] = amax; red[1][wave] = bmax; }
+    if (kt + 1 < nk) store(cur ^ 1);
     __syncthreads();
-    amax = fmaxf(fmaxf(red[0][0], red[0][1]), fmaxf(red[0][2], red[0][3]));
-    bmax = fmaxf(fmaxf(red[1][0], red[1][1]), fmaxf(red[1][2], red[1][3]));
-    // Power-of-two scales putting each chunk panel's largest magnitude in [1, 2).
-    // (a zero, Inf or NaN panel is not scaled, so Inf and NaN reach C as they
-    // are; exponents are clamped to [-126, 127] so that both scales are finite)
-    const int ea = (amax > 0.f && isfinite(amax)) ? max(ilogbf(amax), -126) : 0;
-    const int eb = (bmax > 0.f && isfinite(bmax)) ? max(ilogbf(bmax), -126) : 0;
-    const float sa = ldexpf(1.f, -ea), sb = ldexpf(1.f, -eb);
-    const float unscale = ldexpf(1.f, ea + eb);
+  }
+
+  float* C = Cb + bz * sC;
+  const int rq = (lane >> 4) * 4;
+  const int* ea = eA + bz * (Mp / SPAN);
+  const int* eb = eB + bz * (Np / SPAN);
 #pragma unroll
-    for (int q = 0; q < 32; ++q) {
-      float x = av[q] * sa;
-      __bf16 h = (__bf16)x;
-      As[0][r][kh + q] = h;
-      if constexpr (NP >= 2) {
-        float r1 = x - (float)h;
-        __bf16 mid = (__bf16)r1;
-        As[1][r][kh + q] = mid;
-        if constexpr (NP >= 3) As[2][r][kh + q] = (__bf16)(r1 - (float)mid);
-      }
-      float y = bv[q] * sb;
-      __bf16 hb = (__bf16)y;
-      Bs[0][r][kh + q] = hb;
-      if constexpr (NP >= 2) {
-        float r1 = y - (float)hb;
-        __bf16 mid = (__bf16)r1;
-        Bs[1][r][kh + q] = mid;
-        if constexpr (NP >= 3) Bs[2][r][kh + q] = (__bf16)(r1 - (float)mid);
-      }
-    }
-    __syncthreads();
-    v4f part[4][4];
+  for (int i = 0; i < 4; ++i) {
+    const int64_t rbase = m0 + wm * 64 + i * 16;
+    const int erow = ea[rbase / SPAN];
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) {
+      const int64_t col = n0 + wn * 64 + j * 16 + li;
+      const int es = erow + eb[col / SPAN];
+      if (col >= N) continue;
 #pragma unroll
-      for (int j = 0; j < 4; ++j) part[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
-    const int kp = (kn + 31) & ~31;
-    for (int k0 = 0; k0 < kp; k0 += 32) {
-      v8bf a[NP][4], b[NP][4];
-#pragma unroll
-      for (int pc = 0; pc < NP; ++pc) {
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-          a[pc][i] = *reinterpret_cast<const v8bf*>(&As[pc][wm * 64 + i * 16 + li][k0 + lk]);
-#pragma unroll
-        for (int j = 0; j < 4; ++j)
-          b[pc][j] = *reinterpret_cast<const v8bf*>(&Bs[pc][wn * 64 + j * 16 + li][k0 + lk]);
-      }
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          v4f c = part[i][j];
-          if constexpr (NP == 3) {
-            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2][i], b[0][j], c, 0, 0, 0);
-            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1][i], b[1][j], c, 0, 0, 0);
-            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][i], b[2][j], c, 0, 0, 0);
-          }
-          if constexpr (NP >= 2) {
-            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1][i], b[0][j], c, 0, 0, 0);
-            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][i], b[1][j], c, 0, 0, 0);
-          }
-          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][i], b[0][j], c, 0, 0, 0);
-          part[i][j] = c;
+      for (int v = 0; v < 4; ++v) {
+        const int64_t row = rbase + rq + v;
+        if (row < M) {
+          float* cp = C + row * ldc + col;
+          *cp = *cp - ldexpf(acc[i][j][v], es);
         }
       }
     }
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-      for (int j = 0; j < 4; ++j) acc[i][j] += part[i][j] * unscale;
   }
+}
 
-  const int rq = (lane >> 4) * 4;
-#pragma unroll
-  for (int i = 0; i < 4; ++i) {
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const int64_t row = m0 + wm * 64 + i * 16 + rq + v;
-      if (row >= M) continue;
-      float* crow = C + row * ldc;
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const int64_t col = n0 + wn * 64 + j * 16 + li;
-        if (col < N) crow[col] = crow[col] - acc[i][j][v];
-      }
-    }
-  }
+template <int NP>
+int bf16_update_np(int64_t M, int64_t N, int64_t K, const float* A, int64_t lda, int64_t sA, const float* B,
+                   int64_t ldb, int64_t sB, float* C, int64_t ldc, int64_t sC, int64_t batch, hipStream_t s) {
+  const int64_t Mp = ceil_div(M, TM) * TM, Np = ceil_div(N, TN) * TN;
+  const int Kp = (int)(ceil_div(K, BK) * BK);
+  const size_t ea_bytes = ((size_t)batch * (Mp / SPAN) * sizeof(int) + 255) & ~(size_t)255;
+  const size_t eb_bytes = ((size_t)batch * (Np / SPAN) * sizeof(int) + 255) & ~(size_t)255;
+  char *wa = nullptr, *wb = nullptr;
+  NPPH_TRY(workspace(kWsSplitA, ea_bytes + (size_t)batch * NP * Mp * Kp * sizeof(__bf16), (void**)&wa));
+  NPPH_TRY(workspace(kWsSplitB, eb_bytes + (size_t)batch * NP * Np * Kp * sizeof(__bf16), (void**)&wb));
+  int* eA = reinterpret_cast<int*>(wa);
+  int* eB = reinterpret_cast<int*>(wb);
+  __bf16* PA = reinterpret_cast<__bf16*>(wa + ea_bytes);
+  __bf16* PB = reinterpret_cast<__bf16*>(wb + eb_bytes);
+  hipLaunchKernelGGL(split_rows_kernel<NP>, dim3((unsigned)(Mp / SPAN), (unsigned)batch), dim3(NTH), 0, s, M,
+                     (int)K, Kp, A, lda, sA, PA, Mp, eA);
+  hipLaunchKernelGGL(split_cols_kernel<NP>, dim3((unsigned)(Np / SPAN), (unsigned)batch), dim3(NTH), 0, s, N,
+                     (int)K, Kp, B, ldb, sB, PB, Np, eB);
+  const int tiles_n = (int)(Np / TN);
+  hipLaunchKernelGGL(bf16p_gemm_kernel<NP>, dim3((unsigned)((Mp / TM) * tiles_n), (unsigned)batch), dim3(NTH), 0,
+                     s, M, N, Kp, PA, Mp, eA, PB, Np, eB, C, ldc, sC, tiles_n);
+  NPPH_HIP(hipGetLastError());
+  return NPPH_OK;
 }
 
 }  // namespace
@@ -173,23 +303,21 @@ int gemm_bf16split_update(int64_t M, int64_t N, int64_t K, const float* A, int64
   if (M <= 0 || N <= 0 || batch <= 0) return NPPH_OK;
   if (K <= 0 || K > (1 << 20)) { set_error("bf16 split update: bad K=%ld", (long)K); return NPPH_ERR_INVALID; }
   if (batch > 65535) { set_error("bf16 split update: batch too large"); return NPPH_ERR_INVALID; }
-  const int tiles_m = (int)ceil_div(M, TM), tiles_n = (int)ceil_div(N, TN);
-  dim3 grid((unsigned)(tiles_m * tiles_n), (unsigned)batch);
+  if (ceil_div(M, TM) * ceil_div(N, TN) > (1LL << 31) - 1) {
+    set_error("bf16 split update: too many tiles");
+    return NPPH_ERR_INVALID;
+  }
   ProfScope ps(kProfGemmBf16, 2.0 * M * N * K * batch, 4.0 * batch * ((double)M * K + (double)K * N + 2.0 * M * N), s);
-  if (terms == 1)
-    hipLaunchKernelGGL(bf16split_kernel<1>, grid, dim3(NTH), 0, s, M, N, (int)K, A, lda, sA, B, ldb, sB, C, ldc, sC, tiles_n);
-  else if (terms == 3)
-    hipLaunchKernelGGL(bf16split_kernel<2>, grid, dim3(NTH), 0, s, M, N, (int)K, A, lda, sA, B, ldb, sB, C, ldc, sC, tiles_n);
-  else if (terms == 6)
-    hipLaunchKernelGGL(bf16split_kernel<3>, grid, dim3(NTH), 0, s, M, N, (int)K, A, lda, sA, B, ldb, sB, C, ldc, sC, tiles_n);
-  else { set_error("bf16 split update: terms must be 1, 3 or 6 (got %d)", terms); return NPPH_ERR_INVALID; }
-  NPPH_HIP(hipGetLastError());
-  return NPPH_OK;
+  if (terms == 1) return bf16_update_np<1>(M, N, K, A, lda, sA, B, ldb, sB, C, ldc, sC, batch, s);
+  if (terms == 3) return bf16_update_np<2>(M, N, K, A, lda, sA, B, ldb, sB, C, ldc, sC, batch, s);
+  if (terms == 6) return bf16_update_np<3>(M, N, K, A, lda, sA, B, ldb, sB, C, ldc, sC, batch, s);
+  set_error("bf16 split update: terms must be 1, 3 or 6 (got %d)", terms);
+  return NPPH_ERR_INVALID;
 }
 
-// Device-resident entry for tests: C -= A * B through the split kernel.
 }  // namespace npph
 
+// Device-resident entry for tests: C -= A * B through the split update.
 extern "C" int npph_bf16split_update_dev(int64_t M, int64_t N, int64_t K, const float* A, int64_t lda,
                                          const float* B, int64_t ldb, float* C, int64_t ldc, int terms,
                                          void* stream) {

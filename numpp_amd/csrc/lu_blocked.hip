@@ -203,6 +203,17 @@ __global__ void init_factor_kernel(int64_t count, int n, int32_t* perm, int32_t*
   if (i < count) info[i] = 0;
 }
 
+// Smallest inner dimension of a trailing update that takes the split-BF16 path
+// in the BF16 precision (NUMPP_HIP_BF16_MINK, default 256).
+int bf16_min_k() {
+  static const int v = [] {
+    const char* e = std::getenv("NUMPP_HIP_BF16_MINK");
+    const int x = e ? std::atoi(e) : 256;
+    return x > 0 ? x : 256;
+  }();
+  return v;
+}
+
 // Recursion driver state for one getrf call.
 template <class T>
 struct Rec {
@@ -276,7 +287,10 @@ struct Rec {
   int gemm_update(int64_t M, int64_t N, int64_t K, T* a, T* b, T* c) {
     if (M <= 0 || N <= 0 || K <= 0) return NPPH_OK;
     if constexpr (sizeof(T) == 4) {
-      if (bf16_terms > 0)
+      // split-BF16 for the deep products; the shallow ones (which would pay two
+      // split passes for little MFMA work) stay on the FP32 MFMA path, which is
+      // at least as accurate
+      if (bf16_terms > 0 && K >= bf16_min_k() && M >= 128 && N >= 128)
         return gemm_bf16split_update(M, N, K, (const float*)a, n, strideA, (const float*)b, n, strideA, (float*)c, n,
                                      strideA, count, bf16_terms, s);
     }

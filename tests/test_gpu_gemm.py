@@ -116,9 +116,9 @@ def test_batched_strided_dev(hip):
 
 
 @pytest.mark.parametrize("terms,bits", [(1, 6), (3, 14), (6, 17)])
-def test_bf16_split_update(hip, terms, bits):
-    rng = np.random.default_rng(terms)
-    m, k, n = 300, 64, 260
+@pytest.mark.parametrize("m,k,n", [(300, 64, 260), (257, 1000, 129), (128, 33, 384)])
+def test_bf16_split_update(hip, terms, bits, m, k, n):
+    rng = np.random.default_rng(terms + m)
     a = rng.standard_normal((m, k)).astype(np.float32)
     b = rng.standard_normal((k, n)).astype(np.float32) * 1e-3
     c0 = rng.standard_normal((m, n)).astype(np.float32)

@@ -100,14 +100,16 @@ def test_fp64_precision_reports(hip):
 def test_plain_bf16_falls_back(hip):
     # One-term BF16 (8-bit significand) at kappa = 1e6 cannot refine to 1e-12:
     # the system must stop (stagnation / max_iter) and be re-solved in FP64.
-    a, b = make_stack(8, 2, 200, 2, 1e6)
+    # (n = 1100: the trailing updates with an inner dimension of at least 256
+    # take the split-BF16 path; shallower ones stay in FP32.)
+    a, b = make_stack(8, 2, 1100, 2, 1e6)
     x, berr, iters, info = nla.solve_refined(a, b, precision="bf16", bf16_terms=1, max_iter=4)
     assert np.all(iters == -1), iters
     check_results(a, b, x, berr, cond=1e6)
 
 
 def test_no_fallback_reports_failure(hip):
-    a, b = make_stack(9, 1, 200, 1, 1e6)
+    a, b = make_stack(9, 1, 1100, 1, 1e6)
     x, berr, iters, info = nla.solve_refined(a, b, precision="bf16", bf16_terms=1, max_iter=2, fallback=False)
     assert iters[0] >= 1 and berr[0, 0] > 1e-12
 
