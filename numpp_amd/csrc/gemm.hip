@@ -91,15 +91,32 @@ __global__ __launch_bounds__(NT, 2) void gemm_kernel(GemmArgs<T> p) {
   const int a_row = tid / (BK / AE), a_k = (tid % (BK / AE)) * AE;  // A tile: BM rows x BK
   const int b_k = tid >> 4, b_col = (tid & 15) * BE;  // B tile: BK rows x BN
   const bool mn_full = (m0 + BM <= p.M) && (n0 + BN <= p.N);
+  // physical rows (row maps): this thread's A row, and B rows per K slice
+  const int32_t* mapb = p.maps.b ? p.maps.b + bz * p.maps.stride : nullptr;
+  const int64_t a_phys = p.maps.a ? p.maps.a[bz * p.maps.stride + min(m0 + a_row, p.M - 1)] : m0 + a_row;
+  const int lk = lane >> 4, lr = lane & 15;
+  // the epilogue's C rows, fetched now with the other map entries so that the
+  // kernel pays one map latency, not one at each end
+  int32_t c_phys[IT][4];
+#pragma unroll
+  for (int i = 0; i < IT; ++i)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int64_t row = min(m0 + wm * (BM / 2) + i * 16 + Mfma<T>::row(lk, v), p.M - 1);
+      c_phys[i][v] = p.maps.c ? p.maps.c[bz * p.maps.stride + row] : (int32_t)row;
+    }
 
+  // B's physical row for the slice starting at k0 (a map entry is fetched two
+  // slices ahead of its use, so its latency is not added to the tile loads)
+  auto brow = [&](int64_t k0) -> int64_t { return mapb ? mapb[min(k0 + b_k, p.K - 1)] : k0 + b_k; };
   T ra[AE], rb[BE];
-  auto load = [&](int64_t k0) {
+  auto load = [&](int64_t k0, int64_t b_phys) {
     const bool full = mn_full && (k0 + BK <= p.K) && p.vec;
     const int64_t gr = m0 + a_row, gk = k0 + a_k;
     const int64_t bk = k0 + b_k, bc = n0 + b_col;
     if (full) {
-      const T* pa = A + gr * p.lda + gk;
-      const T* pb = B + bk * p.ldb + bc;
+      const T* pa = A + a_phys * p.lda + gk;
+      const T* pb = B + b_phys * p.ldb + bc;
       if constexpr (sizeof(T) == 8) {
 #pragma unroll
         for (int i = 0; i < AE; i += 2) {
@@ -125,9 +142,9 @@ __global__ __launch_bounds__(NT, 2) void gemm_kernel(GemmArgs<T> p) {
       }
     } else {
 #pragma unroll
-      for (int i = 0; i < AE; ++i) ra[i] = (gr < p.M && gk + i < p.K) ? A[gr * p.lda + gk + i] : T(0);
+      for (int i = 0; i < AE; ++i) ra[i] = (gr < p.M && gk + i < p.K) ? A[a_phys * p.lda + gk + i] : T(0);
 #pragma unroll
-      for (int i = 0; i < BE; ++i) rb[i] = (bk < p.K && bc + i < p.N) ? B[bk * p.ldb + bc + i] : T(0);
+      for (int i = 0; i < BE; ++i) rb[i] = (bk < p.K && bc + i < p.N) ? B[b_phys * p.ldb + bc + i] : T(0);
     }
   };
   auto store = [&](int buf) {
@@ -144,13 +161,19 @@ __global__ __launch_bounds__(NT, 2) void gemm_kernel(GemmArgs<T> p) {
     for (int j = 0; j < JT; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
 
   const int kt_total = (int)((p.K + BK - 1) / BK);
-  load(0);
+  int64_t bnext = brow(BK);
+  load(0, brow(0));
   store(0);
   __syncthreads();
-  const int lk = lane >> 4, lr = lane & 15;
   for (int kt = 0; kt < kt_total; ++kt) {
     const int cur = kt & 1;
-    if (kt + 1 < kt_total) load((int64_t)(kt + 1) * BK);
+    if (kt + 1 < kt_total) {
+      // the map entry for slice kt + 2 is issued before slice kt + 1's tile
+      // loads, so waiting for it later never waits for those (vmcnt is in order)
+      const int64_t b2 = brow((int64_t)(kt + 2) * BK);
+      load((int64_t)(kt + 1) * BK, bnext);
+      bnext = b2;
+    }
 #pragma unroll
     for (int kk = 0; kk < BK; kk += 4) {
       T af[IT], bf[JT];
@@ -175,7 +198,7 @@ __global__ __launch_bounds__(NT, 2) void gemm_kernel(GemmArgs<T> p) {
     for (int v = 0; v < 4; ++v) {
       const int64_t row = m0 + wm * (BM / 2) + i * 16 + Mfma<T>::row(lk, v);
       if (row >= p.M) continue;
-      T* crow = C + row * p.ldc;
+      T* crow = C + (int64_t)c_phys[i][v] * p.ldc;
 #pragma unroll
       for (int j = 0; j < JT; ++j) {
         const int64_t col = n0 + wn * (BN / 2) + j * 16 + lr;
@@ -205,6 +228,10 @@ int gemm_launch(GemmArgs<T> p, int64_t batch, hipStream_t s, int prof) {
     // C = beta * C handled by a K=1 pass with a zero A column is wasteful; the
     // LU and matmul callers never pass K = 0, so treat it as invalid.
     set_error("gemm: K must be positive");
+    return NPPH_ERR_INVALID;
+  }
+  if (p.M >= (1LL << 31)) {  // row indices are kept in 32 bits
+    set_error("gemm: M too large");
     return NPPH_ERR_INVALID;
   }
   if (p.lda < p.K || p.ldb < p.N || p.ldc < p.N) {

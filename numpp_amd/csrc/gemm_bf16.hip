@@ -70,7 +70,7 @@ __device__ __forceinline__ void split_val(float x, __bf16 (&o)[3]) {
 template <int NP>
 __global__ __launch_bounds__(NTH) void split_rows_kernel(int64_t M, int K, int Kp, const float* __restrict__ Ab,
                                                          int64_t lda, int64_t sA, __bf16* __restrict__ P, int64_t Mp,
-                                                         int* __restrict__ eA) {
+                                                         int* __restrict__ eA, RowMaps maps) {
   __shared__ float red[4];
   const int64_t bz = blockIdx.y;
   const int panel = blockIdx.x;
@@ -79,7 +79,8 @@ __global__ __launch_bounds__(NTH) void split_rows_kernel(int64_t M, int K, int K
   const int r = t >> 2, q = t & 3;  // 4 threads per row
   const int64_t row = (int64_t)panel * SPAN + r;
   const bool live = row < M;
-  const float* ar = A + (live ? row : 0) * lda;
+  const int64_t lrow = live ? row : 0;
+  const float* ar = A + (maps.a ? (int64_t)maps.a[bz * maps.stride + lrow] : lrow) * lda;
   float amax = 0.f;
   if (live)
     for (int k = q; k < K; k += 4) amax = fmaxf(amax, fabsf(ar[k]));
@@ -112,19 +113,21 @@ __global__ __launch_bounds__(NTH) void split_rows_kernel(int64_t M, int K, int K
 template <int NP>
 __global__ __launch_bounds__(NTH) void split_cols_kernel(int64_t N, int K, int Kp, const float* __restrict__ Bb,
                                                          int64_t ldb, int64_t sB, __bf16* __restrict__ P, int64_t Np,
-                                                         int* __restrict__ eB) {
+                                                         int* __restrict__ eB, RowMaps maps) {
   __shared__ float red[4];
   __shared__ float tile[SPAN][SPAN + 1];  // [k][col]
   const int64_t bz = blockIdx.y;
   const int panel = blockIdx.x;
   const float* B = Bb + bz * sB;
+  const int32_t* mapb = maps.b ? maps.b + bz * maps.stride : nullptr;
+  auto brow = [&](int k) { return (int64_t)(mapb ? mapb[k] : k) * ldb; };
   const int t = threadIdx.x;
   const int c = t & 63, kr = t >> 6;  // read: column c, rows kr, kr + 4, ...
   const int64_t col = (int64_t)panel * SPAN + c;
   const bool live = col < N;
   float amax = 0.f;
   if (live)
-    for (int k = kr; k < K; k += 4) amax = fmaxf(amax, fabsf(B[(int64_t)k * ldb + col]));
+    for (int k = kr; k < K; k += 4) amax = fmaxf(amax, fabsf(B[brow(k) + col]));
   amax = block_max(amax, red);
   const int e = panel_exp(amax);
   if (t == 0) eB[bz * (Np / SPAN) + panel] = e;
@@ -136,7 +139,7 @@ __global__ __launch_bounds__(NTH) void split_cols_kernel(int64_t N, int K, int K
 #pragma unroll
     for (int u = 0; u < SPAN / 4; ++u) {
       const int k = kb + kr + 4 * u;
-      const float b = B[(int64_t)min(k, K - 1) * ldb + (live ? col : 0)];  // clamped
+      const float b = B[brow(min(k, K - 1)) + (live ? col : 0)];  // clamped
       tile[kr + 4 * u][c] = (live && k < K) ? b * sc : 0.f;
     }
     __syncthreads();
@@ -167,7 +170,7 @@ __global__ __launch_bounds__(NTH) void bf16p_gemm_kernel(int64_t M, int64_t N, i
                                                          int64_t Mp, const int* __restrict__ eA,
                                                          const __bf16* __restrict__ PB, int64_t Np,
                                                          const int* __restrict__ eB, float* __restrict__ Cb,
-                                                         int64_t ldc, int64_t sC, int tiles_n) {
+                                                         int64_t ldc, int64_t sC, int tiles_n, RowMaps maps) {
   __shared__ __bf16 As[2][NP][TM][KPAD];
   __shared__ __bf16 Bs[2][NP][TN][KPAD];
   const int64_t bz = blockIdx.y;
@@ -262,7 +265,7 @@ __global__ __launch_bounds__(NTH) void bf16p_gemm_kernel(int64_t M, int64_t N, i
       for (int v = 0; v < 4; ++v) {
         const int64_t row = rbase + rq + v;
         if (row < M) {
-          float* cp = C + row * ldc + col;
+          float* cp = C + (maps.c ? (int64_t)maps.c[bz * maps.stride + row] : row) * ldc + col;
           *cp = *cp - ldexpf(acc[i][j][v], es);
         }
       }
@@ -272,7 +275,8 @@ __global__ __launch_bounds__(NTH) void bf16p_gemm_kernel(int64_t M, int64_t N, i
 
 template <int NP>
 int bf16_update_np(int64_t M, int64_t N, int64_t K, const float* A, int64_t lda, int64_t sA, const float* B,
-                   int64_t ldb, int64_t sB, float* C, int64_t ldc, int64_t sC, int64_t batch, hipStream_t s) {
+                   int64_t ldb, int64_t sB, float* C, int64_t ldc, int64_t sC, int64_t batch, hipStream_t s,
+                   const RowMaps& maps) {
   const int64_t Mp = ceil_div(M, TM) * TM, Np = ceil_div(N, TN) * TN;
   const int Kp = (int)(ceil_div(K, BK) * BK);
   const size_t ea_bytes = ((size_t)batch * (Mp / SPAN) * sizeof(int) + 255) & ~(size_t)255;
@@ -285,12 +289,12 @@ int bf16_update_np(int64_t M, int64_t N, int64_t K, const float* A, int64_t lda,
   __bf16* PA = reinterpret_cast<__bf16*>(wa + ea_bytes);
   __bf16* PB = reinterpret_cast<__bf16*>(wb + eb_bytes);
   hipLaunchKernelGGL(split_rows_kernel<NP>, dim3((unsigned)(Mp / SPAN), (unsigned)batch), dim3(NTH), 0, s, M,
-                     (int)K, Kp, A, lda, sA, PA, Mp, eA);
+                     (int)K, Kp, A, lda, sA, PA, Mp, eA, maps);
   hipLaunchKernelGGL(split_cols_kernel<NP>, dim3((unsigned)(Np / SPAN), (unsigned)batch), dim3(NTH), 0, s, N,
-                     (int)K, Kp, B, ldb, sB, PB, Np, eB);
+                     (int)K, Kp, B, ldb, sB, PB, Np, eB, maps);
   const int tiles_n = (int)(Np / TN);
   hipLaunchKernelGGL(bf16p_gemm_kernel<NP>, dim3((unsigned)((Mp / TM) * tiles_n), (unsigned)batch), dim3(NTH), 0,
-                     s, M, N, Kp, PA, Mp, eA, PB, Np, eB, C, ldc, sC, tiles_n);
+                     s, M, N, Kp, PA, Mp, eA, PB, Np, eB, C, ldc, sC, tiles_n, maps);
   NPPH_HIP(hipGetLastError());
   return NPPH_OK;
 }
@@ -299,7 +303,7 @@ int bf16_update_np(int64_t M, int64_t N, int64_t K, const float* A, int64_t lda,
 
 int gemm_bf16split_update(int64_t M, int64_t N, int64_t K, const float* A, int64_t lda, int64_t sA,
                           const float* B, int64_t ldb, int64_t sB, float* C, int64_t ldc, int64_t sC,
-                          int64_t batch, int terms, hipStream_t s) {
+                          int64_t batch, int terms, hipStream_t s, const RowMaps& maps) {
   if (M <= 0 || N <= 0 || batch <= 0) return NPPH_OK;
   if (K <= 0 || K > (1 << 20)) { set_error("bf16 split update: bad K=%ld", (long)K); return NPPH_ERR_INVALID; }
   if (batch > 65535) { set_error("bf16 split update: batch too large"); return NPPH_ERR_INVALID; }
@@ -308,9 +312,9 @@ int gemm_bf16split_update(int64_t M, int64_t N, int64_t K, const float* A, int64
     return NPPH_ERR_INVALID;
   }
   ProfScope ps(kProfGemmBf16, 2.0 * M * N * K * batch, 4.0 * batch * ((double)M * K + (double)K * N + 2.0 * M * N), s);
-  if (terms == 1) return bf16_update_np<1>(M, N, K, A, lda, sA, B, ldb, sB, C, ldc, sC, batch, s);
-  if (terms == 3) return bf16_update_np<2>(M, N, K, A, lda, sA, B, ldb, sB, C, ldc, sC, batch, s);
-  if (terms == 6) return bf16_update_np<3>(M, N, K, A, lda, sA, B, ldb, sB, C, ldc, sC, batch, s);
+  if (terms == 1) return bf16_update_np<1>(M, N, K, A, lda, sA, B, ldb, sB, C, ldc, sC, batch, s, maps);
+  if (terms == 3) return bf16_update_np<2>(M, N, K, A, lda, sA, B, ldb, sB, C, ldc, sC, batch, s, maps);
+  if (terms == 6) return bf16_update_np<3>(M, N, K, A, lda, sA, B, ldb, sB, C, ldc, sC, batch, s, maps);
   set_error("bf16 split update: terms must be 1, 3 or 6 (got %d)", terms);
   return NPPH_ERR_INVALID;
 }

@@ -51,9 +51,9 @@ __device__ __forceinline__ int mfma_row(int lk, int v) { return sizeof(T) == 4 ?
 // backward sweep adds x / scale to the FP64 iterate (in place of storing X).
 template <class T, bool LOWER, int NT, int CW, bool REF>
 __global__ __launch_bounds__(NT) void sweep_kernel(int n, int nrhs, const T* LUb, int64_t strideA,
-                                                   const int32_t* permb, const T* Bb, int64_t strideB, T* Xb,
-                                                   int64_t strideX, const int32_t* info, const int32_t* mask,
-                                                   RefIO ref) {
+                                                   const int32_t* permb, const int32_t* rowmapb, const T* Bb,
+                                                   int64_t strideB, T* Xb, int64_t strideX, const int32_t* info,
+                                                   const int32_t* mask, RefIO ref) {
   constexpr int S = NT / GB;       // threads per row in the off-diagonal pass
   constexpr int NWV = NT / 64;     // waves
   constexpr int CPW = (CW + NWV - 1) / NWV;  // columns per wave in the diagonal solve
@@ -68,6 +68,9 @@ __global__ __launch_bounds__(NT) void sweep_kernel(int n, int nrhs, const T* LUb
   const int c0 = (int)blockIdx.y * CW, nc = min(CW, nrhs - c0);
   const T* __restrict__ LU = LUb + mat * strideA;
   T* __restrict__ X = Xb + mat * strideX;
+  // factor row of position p (factors whose rows never moved carry a row map)
+  const int32_t* rowmap = rowmapb ? rowmapb + mat * n : nullptr;
+  auto rowp = [&](int p) -> int64_t { return rowmap ? rowmap[p] : p; };
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   extern __shared__ unsigned char smem_raw[];
   T* xs = reinterpret_cast<T*>(smem_raw);  // n x CW
@@ -134,7 +137,7 @@ __global__ __launch_bounds__(NT) void sweep_kernel(int n, int nrhs, const T* LUb
 #pragma unroll
     for (int u = 0; u < DPT; ++u) {
       const int e = t + u * NT, rr = e / GB, cc = e - rr * GB;
-      dpre[u] = LU[(int64_t)(q0 + min(rr, qs - 1)) * n + q0 + min(cc, qs - 1)];
+      dpre[u] = LU[rowp(q0 + min(rr, qs - 1)) * n + q0 + min(cc, qs - 1)];
     }
   };
   load_diag(0);
@@ -154,7 +157,7 @@ __global__ __launch_bounds__(NT) void sweep_kernel(int n, int nrhs, const T* LUb
       const int lr = lane & 15, lk = lane >> 4, rw = 16 * wave;
       acc_t acc = {};
       if (rw < bs && klo < khi) {
-        const T* __restrict__ Lr = LU + (int64_t)(r0 + min(rw + lr, bs - 1)) * n;
+        const T* __restrict__ Lr = LU + rowp(r0 + min(rw + lr, bs - 1)) * n;
         // aligned vector loads when every row starts on 16 bytes
         const bool vec = n % (16 / (int)sizeof(T)) == 0 && n >= 4 && reinterpret_cast<uintptr_t>(LUb) % 16 == 0 &&
                          strideA % (16 / (int)sizeof(T)) == 0;
@@ -203,7 +206,7 @@ __global__ __launch_bounds__(NT) void sweep_kernel(int n, int nrhs, const T* LUb
 #pragma unroll
     for (int c = 0; c < CW; ++c) acc[c] = T(0);
     if (r < bs && klo < khi) {
-      const T* __restrict__ Lr = LU + (int64_t)(r0 + r) * n;
+      const T* __restrict__ Lr = LU + rowp(r0 + r) * n;
       // M entries per thread in flight. Indices past the range are clamped
       // and their factor entry zeroed, so neither the loads nor the FMAs need
       // a branch (a clamped slice row is a solved, finite value).
@@ -383,8 +386,8 @@ int chunk_width(int64_t count, int n, int nrhs) {
 // instantiation.
 template <class T, bool LOWER, int NT, int CW, bool REF>
 int launch_sweep_cw(dim3 grid, size_t shm, hipStream_t s, int n, int nrhs, const T* LU, int64_t strideA,
-                    const int32_t* perm, const T* B, int64_t strideB, T* X, int64_t strideX, const int32_t* info,
-                    const int32_t* mask, const RefIO& ref) {
+                    const int32_t* perm, const int32_t* rowmap, const T* B, int64_t strideB, T* X, int64_t strideX,
+                    const int32_t* info, const int32_t* mask, const RefIO& ref) {
   auto* k = sweep_kernel<T, LOWER, NT, CW, REF>;
   // static LDS (diagonal block, partials) plus the slice can pass 64 KiB:
   // opt in once per instantiation to the full dynamic budget
@@ -392,17 +395,18 @@ int launch_sweep_cw(dim3 grid, size_t shm, hipStream_t s, int n, int nrhs, const
       reinterpret_cast<const void*>(k), hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_budget());
   NPPH_HIP(once);
   (void)shm;
-  hipLaunchKernelGGL(k, grid, dim3(NT), shm, s, n, nrhs, LU, strideA, perm, B, strideB, X, strideX, info, mask, ref);
+  hipLaunchKernelGGL(k, grid, dim3(NT), shm, s, n, nrhs, LU, strideA, perm, rowmap, B, strideB, X, strideX, info, mask,
+                     ref);
   return NPPH_OK;
 }
 
 template <class T, bool LOWER, int NT, bool REF>
 int launch_sweep(int cw, dim3 grid, size_t shm, hipStream_t s, int n, int nrhs, const T* LU, int64_t strideA,
-                 const int32_t* perm, const T* B, int64_t strideB, T* X, int64_t strideX, const int32_t* info,
-                 const int32_t* mask, const RefIO& ref) {
-#define NPPH_SWEEP(CWV)                                                                                         \
-  return launch_sweep_cw<T, LOWER, NT, CWV, REF>(grid, shm, s, n, nrhs, LU, strideA, perm, B, strideB, X, strideX, \
-                                                 info, mask, ref)
+                 const int32_t* perm, const int32_t* rowmap, const T* B, int64_t strideB, T* X, int64_t strideX,
+                 const int32_t* info, const int32_t* mask, const RefIO& ref) {
+#define NPPH_SWEEP(CWV)                                                                                        \
+  return launch_sweep_cw<T, LOWER, NT, CWV, REF>(grid, shm, s, n, nrhs, LU, strideA, perm, rowmap, B, strideB, X, \
+                                                 strideX, info, mask, ref)
   if constexpr (sizeof(T) == 4) {
     if (cw == 16) NPPH_SWEEP(16);
   }
@@ -423,17 +427,17 @@ bool getrs_mw_off() {
 }
 
 template <class T, bool REF>
-int getrs_impl(int64_t count, int n, int nrhs, const T* LU, int64_t strideA, const int32_t* perm, const T* B,
-               int64_t strideB, T* X, int64_t strideX, const int32_t* info, const int32_t* mask, const RefIO& ref,
-               hipStream_t s) {
+int getrs_impl(int64_t count, int n, int nrhs, const T* LU, int64_t strideA, const int32_t* perm,
+               const int32_t* rowmap, const T* B, int64_t strideB, T* X, int64_t strideX, const int32_t* info,
+               const int32_t* mask, const RefIO& ref, hipStream_t s) {
   if (count <= 0 || n <= 0 || nrhs <= 0) return NPPH_OK;
   if (!REF && B == X) { set_error("getrs: B and X must not alias"); return NPPH_ERR_INVALID; }
   if (count > 2147483647LL) { set_error("getrs: count too large"); return NPPH_ERR_INVALID; }
   if (getrs_mw_wanted(count, n, nrhs) && !getrs_mw_off()) {
     ProfScope ps(kProfGetrs, 2.0 * count * n * n * nrhs,
                  (double)count * (sizeof(T) * (double)n * n + 3.0 * sizeof(T) * n * nrhs), s);
-    return getrs_mw<T, REF>(count, n, nrhs, LU, strideA, perm, B, strideB, X, strideX, info, mask, ref.R, ref.scale,
-                            ref.X, ref.first, s);
+    return getrs_mw<T, REF>(count, n, nrhs, LU, strideA, perm, rowmap, B, strideB, X, strideX, info, mask, ref.R,
+                            ref.scale, ref.X, ref.first, s);
   }
   const int cw = chunk_width<T>(count, n, nrhs);
   const int nz = (int)ceil_div(nrhs, cw);
@@ -443,15 +447,15 @@ int getrs_impl(int64_t count, int n, int nrhs, const T* LU, int64_t strideA, con
   ProfScope ps(kProfGetrs, 2.0 * count * n * n * nrhs,
                (double)count * (sizeof(T) * (double)n * n + 3.0 * sizeof(T) * n * nrhs), s);
   if (n <= 256) {
-    NPPH_TRY((launch_sweep<T, true, 256, REF>(cw, grid, shm, s, n, nrhs, LU, strideA, perm, B, strideB, X, strideX,
-                                              info, mask, ref)));
-    NPPH_TRY((launch_sweep<T, false, 256, REF>(cw, grid, shm, s, n, nrhs, LU, strideA, perm, B, strideB, X, strideX,
-                                               info, mask, ref)));
+    NPPH_TRY((launch_sweep<T, true, 256, REF>(cw, grid, shm, s, n, nrhs, LU, strideA, perm, rowmap, B, strideB, X,
+                                              strideX, info, mask, ref)));
+    NPPH_TRY((launch_sweep<T, false, 256, REF>(cw, grid, shm, s, n, nrhs, LU, strideA, perm, rowmap, B, strideB, X,
+                                               strideX, info, mask, ref)));
   } else {
-    NPPH_TRY((launch_sweep<T, true, 512, REF>(cw, grid, shm, s, n, nrhs, LU, strideA, perm, B, strideB, X, strideX,
-                                              info, mask, ref)));
-    NPPH_TRY((launch_sweep<T, false, 512, REF>(cw, grid, shm, s, n, nrhs, LU, strideA, perm, B, strideB, X, strideX,
-                                               info, mask, ref)));
+    NPPH_TRY((launch_sweep<T, true, 512, REF>(cw, grid, shm, s, n, nrhs, LU, strideA, perm, rowmap, B, strideB, X,
+                                              strideX, info, mask, ref)));
+    NPPH_TRY((launch_sweep<T, false, 512, REF>(cw, grid, shm, s, n, nrhs, LU, strideA, perm, rowmap, B, strideB, X,
+                                               strideX, info, mask, ref)));
   }
   NPPH_HIP(hipGetLastError());
   return NPPH_OK;
@@ -460,22 +464,25 @@ int getrs_impl(int64_t count, int n, int nrhs, const T* LU, int64_t strideA, con
 template <class T>
 int blocked_getrs_perm(int64_t count, int n, int nrhs, const T* LU, int64_t strideA, const int32_t* perm,
                        const T* B, int64_t strideB, T* X, int64_t strideX, const int32_t* info, const int32_t* mask,
-                       hipStream_t s) {
-  return getrs_impl<T, false>(count, n, nrhs, LU, strideA, perm, B, strideB, X, strideX, info, mask, RefIO{}, s);
+                       hipStream_t s, const int32_t* rowmap) {
+  return getrs_impl<T, false>(count, n, nrhs, LU, strideA, perm, rowmap, B, strideB, X, strideX, info, mask, RefIO{},
+                              s);
 }
 
 int getrs_refine(int64_t count, int n, int nrhs, const float* LU, int64_t strideA, const int32_t* perm,
                  const double* R, const double* scale, float* Y, double* X, int first, const int32_t* mask,
-                 hipStream_t s) {
+                 hipStream_t s, const int32_t* rowmap) {
   const int64_t per = (int64_t)n * nrhs;
-  return getrs_impl<float, true>(count, n, nrhs, LU, strideA, perm, nullptr, per, Y, per, nullptr, mask,
+  return getrs_impl<float, true>(count, n, nrhs, LU, strideA, perm, rowmap, nullptr, per, Y, per, nullptr, mask,
                                  RefIO{R, scale, X, first}, s);
 }
 
 template int blocked_getrs_perm<double>(int64_t, int, int, const double*, int64_t, const int32_t*, const double*,
-                                        int64_t, double*, int64_t, const int32_t*, const int32_t*, hipStream_t);
+                                        int64_t, double*, int64_t, const int32_t*, const int32_t*, hipStream_t,
+                                        const int32_t*);
 template int blocked_getrs_perm<float>(int64_t, int, int, const float*, int64_t, const int32_t*, const float*,
-                                       int64_t, float*, int64_t, const int32_t*, const int32_t*, hipStream_t);
+                                       int64_t, float*, int64_t, const int32_t*, const int32_t*, hipStream_t,
+                                       const int32_t*);
 
 int ipiv_to_perm(int64_t count, int n, const int32_t* ipiv, int32_t* perm, hipStream_t s) {
   if (count <= 0 || n <= 0) return NPPH_OK;

@@ -80,7 +80,8 @@ __global__ void mw_clear_kernel(uint32_t* sync, int64_t words) {
 template <class T, bool LOWER, int CW, bool REF>
 __global__ __launch_bounds__(MT) void sweep_mw_kernel(int64_t count, int n, int nrhs, int nz,
                                                       const T* __restrict__ LUb, int64_t strideA,
-                                                      const int32_t* __restrict__ permb, const T* __restrict__ Bb,
+                                                      const int32_t* __restrict__ permb,
+                                                      const int32_t* __restrict__ rowmapb, const T* __restrict__ Bb,
                                                       int64_t strideB, T* Xb, int64_t strideX,
                                                       const int32_t* __restrict__ info,
                                                       const int32_t* __restrict__ mask, MwRef ref, uint32_t* sync) {
@@ -109,6 +110,9 @@ __global__ __launch_bounds__(MT) void sweep_mw_kernel(int64_t count, int n, int 
   const int c0 = z * CW, nc = min(CW, nrhs - c0);
   const T* __restrict__ LU = LUb + mat * strideA;
   T* X = Xb + mat * strideX;
+  // factor row of position p (factors whose rows never moved carry a row map)
+  const int32_t* rowmap = rowmapb ? rowmapb + mat * n : nullptr;
+  auto rowp = [&](int p) -> int64_t { return rowmap ? rowmap[p] : p; };
   uint32_t* flags = sync + kSyncHead + (mat * nz + z) * nblk;
   const uint32_t ready = LOWER ? 1u : 2u;
 
@@ -119,7 +123,7 @@ __global__ __launch_bounds__(MT) void sweep_mw_kernel(int64_t count, int n, int 
 #pragma unroll
     for (int u = 0; u < DPT; ++u) {
       const int e = t + u * MT, rr = e / MB, cc = e - rr * MB;
-      d[u] = LU[(int64_t)(r0 + min(rr, bs - 1)) * n + r0 + min(cc, bs - 1)];
+      d[u] = LU[rowp(r0 + min(rr, bs - 1)) * n + r0 + min(cc, bs - 1)];
     }
 #pragma unroll
     for (int u = 0; u < DPT; ++u) {
@@ -155,7 +159,7 @@ __global__ __launch_bounds__(MT) void sweep_mw_kernel(int64_t count, int n, int 
 
   // ---- products with the blocks solved before this one
   const int lr = lane & 15, lk = lane >> 4, rw = 16 * wave;
-  const T* __restrict__ Lrow = LU + (int64_t)(r0 + min(rw + lr, bs - 1)) * n;
+  const T* __restrict__ Lrow = LU + rowp(r0 + min(rw + lr, bs - 1)) * n;
   const bool vec = (n % 4 == 0) && (reinterpret_cast<uintptr_t>(LUb) % 16 == 0) && (strideA % 4 == 0);
   // a[u][q] = factor entry (row rw + lr, column q0 + 16 u + 4 lk + q) of
   // dependency block j; columns past the block are zeroed
@@ -300,13 +304,13 @@ __global__ __launch_bounds__(MT) void sweep_mw_kernel(int64_t count, int n, int 
 
 template <class T, int CW, bool REF>
 void launch_mw(int64_t count, int n, int nrhs, int nz, const T* LU, int64_t strideA, const int32_t* perm,
-               const T* B, int64_t strideB, T* X, int64_t strideX, const int32_t* info, const int32_t* mask,
-               const MwRef& ref, uint32_t* sync, hipStream_t s) {
+               const int32_t* rowmap, const T* B, int64_t strideB, T* X, int64_t strideX, const int32_t* info,
+               const int32_t* mask, const MwRef& ref, uint32_t* sync, hipStream_t s) {
   const int64_t wgs = count * nz * ceil_div(n, MB);
   hipLaunchKernelGGL((sweep_mw_kernel<T, true, CW, REF>), dim3((unsigned)wgs), dim3(MT), 0, s, count, n, nrhs, nz,
-                     LU, strideA, perm, B, strideB, X, strideX, info, mask, ref, sync);
+                     LU, strideA, perm, rowmap, B, strideB, X, strideX, info, mask, ref, sync);
   hipLaunchKernelGGL((sweep_mw_kernel<T, false, CW, REF>), dim3((unsigned)wgs), dim3(MT), 0, s, count, n, nrhs, nz,
-                     LU, strideA, perm, B, strideB, X, strideX, info, mask, ref, sync);
+                     LU, strideA, perm, rowmap, B, strideB, X, strideX, info, mask, ref, sync);
 }
 
 }  // namespace
@@ -320,9 +324,9 @@ bool getrs_mw_wanted(int64_t count, int n, int nrhs) {
 }
 
 template <class T, bool REF>
-int getrs_mw(int64_t count, int n, int nrhs, const T* LU, int64_t strideA, const int32_t* perm, const T* B,
-             int64_t strideB, T* X, int64_t strideX, const int32_t* info, const int32_t* mask, const double* R,
-             const double* scale, double* X64, int first, hipStream_t s) {
+int getrs_mw(int64_t count, int n, int nrhs, const T* LU, int64_t strideA, const int32_t* perm, const int32_t* rowmap,
+             const T* B, int64_t strideB, T* X, int64_t strideX, const int32_t* info, const int32_t* mask,
+             const double* R, const double* scale, double* X64, int first, hipStream_t s) {
   const int cw = nrhs <= 4 ? 4 : (nrhs <= 8 ? 8 : 16);
   const int nz = (int)ceil_div(nrhs, cw);
   const int64_t words = kSyncHead + count * nz * ceil_div(n, MB);
@@ -330,21 +334,25 @@ int getrs_mw(int64_t count, int n, int nrhs, const T* LU, int64_t strideA, const
   NPPH_TRY(workspace(kWsSync, sizeof(uint32_t) * (size_t)words, (void**)&sync));
   hipLaunchKernelGGL(mw_clear_kernel, dim3((unsigned)ceil_div(words, 256)), dim3(256), 0, s, sync, words);
   MwRef ref{R, scale, X64, first};
-  if (cw == 4) launch_mw<T, 4, REF>(count, n, nrhs, nz, LU, strideA, perm, B, strideB, X, strideX, info, mask, ref, sync, s);
-  else if (cw == 8) launch_mw<T, 8, REF>(count, n, nrhs, nz, LU, strideA, perm, B, strideB, X, strideX, info, mask, ref, sync, s);
-  else launch_mw<T, 16, REF>(count, n, nrhs, nz, LU, strideA, perm, B, strideB, X, strideX, info, mask, ref, sync, s);
+  if (cw == 4)
+    launch_mw<T, 4, REF>(count, n, nrhs, nz, LU, strideA, perm, rowmap, B, strideB, X, strideX, info, mask, ref, sync, s);
+  else if (cw == 8)
+    launch_mw<T, 8, REF>(count, n, nrhs, nz, LU, strideA, perm, rowmap, B, strideB, X, strideX, info, mask, ref, sync, s);
+  else
+    launch_mw<T, 16, REF>(count, n, nrhs, nz, LU, strideA, perm, rowmap, B, strideB, X, strideX, info, mask, ref, sync,
+                          s);
   NPPH_HIP(hipGetLastError());
   return NPPH_OK;
 }
 
-template int getrs_mw<double, false>(int64_t, int, int, const double*, int64_t, const int32_t*, const double*,
-                                     int64_t, double*, int64_t, const int32_t*, const int32_t*, const double*,
-                                     const double*, double*, int, hipStream_t);
-template int getrs_mw<float, false>(int64_t, int, int, const float*, int64_t, const int32_t*, const float*, int64_t,
-                                    float*, int64_t, const int32_t*, const int32_t*, const double*, const double*,
-                                    double*, int, hipStream_t);
-template int getrs_mw<float, true>(int64_t, int, int, const float*, int64_t, const int32_t*, const float*, int64_t,
-                                   float*, int64_t, const int32_t*, const int32_t*, const double*, const double*,
-                                   double*, int, hipStream_t);
+template int getrs_mw<double, false>(int64_t, int, int, const double*, int64_t, const int32_t*, const int32_t*,
+                                     const double*, int64_t, double*, int64_t, const int32_t*, const int32_t*,
+                                     const double*, const double*, double*, int, hipStream_t);
+template int getrs_mw<float, false>(int64_t, int, int, const float*, int64_t, const int32_t*, const int32_t*,
+                                    const float*, int64_t, float*, int64_t, const int32_t*, const int32_t*,
+                                    const double*, const double*, double*, int, hipStream_t);
+template int getrs_mw<float, true>(int64_t, int, int, const float*, int64_t, const int32_t*, const int32_t*,
+                                   const float*, int64_t, float*, int64_t, const int32_t*, const int32_t*,
+                                   const double*, const double*, double*, int, hipStream_t);
 
 }  // namespace npph

@@ -35,6 +35,7 @@ int sgesv_dev(int n, int nrhs, float* A, float* B, int32_t* info, hipStream_t s)
   NPPH_TRY(workspace(kWsPerm, sizeof(int32_t) * (size_t)n, (void**)&perm));
   NPPH_TRY(workspace(kWsD, sizeof(float) * (size_t)n * nrhs, (void**)&X));
   NPPH_TRY(blocked_getrf_perm<float>(1, n, A, (int64_t)n * n, ipiv, perm, info, 0, nullptr, s));
+  NPPH_TRY(unscramble_rows<float>(1, n, A, (int64_t)n * n, perm, s));
   NPPH_TRY(blocked_getrs_perm<float>(1, n, nrhs, A, (int64_t)n * n, perm, B, (int64_t)n * nrhs, X,
                                      (int64_t)n * nrhs, info, nullptr, s));
   NPPH_HIP(hipMemcpyAsync(B, X, sizeof(float) * (size_t)n * nrhs, hipMemcpyDeviceToDevice, s));
@@ -89,6 +90,7 @@ int npph_lapack_gesv(int n, int nrhs, int dtype, void* A, void* B) {
     NPPH_TRY(workspace(kWsPiv, sizeof(int32_t) * (size_t)n, (void**)&ipiv));
     NPPH_TRY(workspace(kWsPerm, sizeof(int32_t) * (size_t)n, (void**)&perm));
     NPPH_TRY(blocked_getrf_perm<float>(1, n, (float*)dA, (int64_t)na, ipiv, perm, dinfo, 0, nullptr, s));
+    NPPH_TRY(unscramble_rows<float>(1, n, (float*)dA, (int64_t)na, perm, s));
   }
   int32_t info = 0;
   NPPH_HIP(hipMemcpyAsync(&info, dinfo, sizeof(int32_t), hipMemcpyDeviceToHost, s));

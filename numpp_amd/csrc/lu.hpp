@@ -47,21 +47,31 @@ size_t blocked_getrf_scratch_bytes(int64_t count, int n, size_t elem);
 
 // Blocked right-looking LU in working precision T (double or float) of a
 // uniform stack stored in place (count x n x n, element stride strideA).
-// ipiv: count x n (0-based pivot positions), perm: count x n (original row at
-// each final position), info: count (0 or first zero-pivot step + 1).
+// Rows never move: on return logical row p of L\U is physical row perm[p]
+// (perm: count x n, the original row at each final position); ipiv: count x n
+// (0-based pivot positions), info: count (0 or first zero-pivot step + 1).
 // bf16_terms > 0 selects the split-BF16 MFMA trailing update (T = float only).
 // mask (nullable, count): matrices with mask == 0 are skipped.
 template <class T>
 int blocked_getrf_perm(int64_t count, int n, T* A, int64_t strideA, int32_t* ipiv, int32_t* perm,
                        int32_t* info, int bf16_terms, const int32_t* mask, hipStream_t s);
+// Packed row order (row p = physical row perm[p]) of factors from
+// blocked_getrf_perm, in place.
+template <class T>
+int unscramble_rows(int64_t count, int n, T* A, int64_t strideA, const int32_t* perm, hipStream_t s);
+// Factors of the refined solver keep the blocked LU's row map, except FP64
+// factors of order <= 64 (the one-wave kernel writes them packed).
+inline bool factor_rows_mapped(int precision, int n) { return !(precision == NPPH_PREC_FP64 && n <= 64); }
 
-// X = U^-1 L^-1 P B with packed factors in T and the permutation from the
-// factorisation. B and X must not alias. Systems with info != 0 (when info is
+// X = U^-1 L^-1 P B with factors in T and the permutation from the
+// factorisation; logical factor row p is physical row rowmap[p] (rowmap =
+// perm for blocked_getrf_perm's output) or row p when rowmap is null (packed
+// factors). B and X must not alias. Systems with info != 0 (when info is
 // non-null) or mask == 0 (when mask is non-null) are skipped.
 template <class T>
 int blocked_getrs_perm(int64_t count, int n, int nrhs, const T* LU, int64_t strideA, const int32_t* perm,
                        const T* B, int64_t strideB, T* X, int64_t strideX, const int32_t* info,
-                       const int32_t* mask, hipStream_t s);
+                       const int32_t* mask, hipStream_t s, const int32_t* rowmap = nullptr);
 
 // One refinement correction with FP32 factors: D = U^-1 L^-1 P fp32(R * scale)
 // (per-column scale), then X = D / scale (first) or X += D / scale, in two
@@ -69,7 +79,7 @@ int blocked_getrs_perm(int64_t count, int n, int nrhs, const T* LU, int64_t stri
 // systems with mask == 1 (still iterating) are corrected.
 int getrs_refine(int64_t count, int n, int nrhs, const float* LU, int64_t strideA, const int32_t* perm,
                  const double* R, const double* scale, float* Y, double* X, int first, const int32_t* mask,
-                 hipStream_t s);
+                 hipStream_t s, const int32_t* rowmap);
 
 // Sweeps with many workgroups per system (getrs_mw.hip): wanted for stacks of
 // few systems of order >= 512, where one workgroup per system and column chunk
@@ -78,9 +88,9 @@ int getrs_refine(int64_t count, int n, int nrhs, const float* LU, int64_t stride
 // X64; X then holds the FP32 forward result and correction).
 bool getrs_mw_wanted(int64_t count, int n, int nrhs);
 template <class T, bool REF>
-int getrs_mw(int64_t count, int n, int nrhs, const T* LU, int64_t strideA, const int32_t* perm, const T* B,
-             int64_t strideB, T* X, int64_t strideX, const int32_t* info, const int32_t* mask, const double* R,
-             const double* scale, double* X64, int first, hipStream_t s);
+int getrs_mw(int64_t count, int n, int nrhs, const T* LU, int64_t strideA, const int32_t* perm, const int32_t* rowmap,
+             const T* B, int64_t strideB, T* X, int64_t strideX, const int32_t* info, const int32_t* mask,
+             const double* R, const double* scale, double* X64, int first, hipStream_t s);
 
 // perm from LAPACK-style pivots (replays the swaps).
 int ipiv_to_perm(int64_t count, int n, const int32_t* ipiv, int32_t* perm, hipStream_t s);

@@ -25,11 +25,14 @@ int gesv_dev_impl(int64_t count, int n, int nrhs, double* A, int64_t strideA, do
   int32_t* perm;
   NPPH_TRY(workspace(kWsPerm, sizeof(int32_t) * (size_t)count * n, (void**)&perm));
   NPPH_TRY(blocked_getrf_perm<double>(count, n, A, strideA, ip, perm, inf, 0, nullptr, s));
-  if (nrhs <= 0) return NPPH_OK;
+  // the factorisation leaves logical row p in physical row perm[p]: solve
+  // through the map, then return the factors in row order
+  if (nrhs <= 0) return unscramble_rows<double>(count, n, A, strideA, perm, s);
   double* X;
   const int64_t sx = (int64_t)n * nrhs;
   NPPH_TRY(workspace(kWsX, sizeof(double) * (size_t)(count * sx), (void**)&X));
-  NPPH_TRY(blocked_getrs_perm<double>(count, n, nrhs, A, strideA, perm, B, strideB, X, sx, inf, nullptr, s));
+  NPPH_TRY(blocked_getrs_perm<double>(count, n, nrhs, A, strideA, perm, B, strideB, X, sx, inf, nullptr, s, perm));
+  NPPH_TRY(unscramble_rows<double>(count, n, A, strideA, perm, s));
   // Singular systems (info != 0) are skipped by getrs; their B content on
   // return is unspecified, as the reference raises instead of returning one.
   if (strideB == sx) {

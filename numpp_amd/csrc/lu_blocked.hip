@@ -2,21 +2,22 @@
 // in working precision T (double for the plain path, float for the
 // mixed-precision path).
 //
-// getrf_rec(k0, w) factors the column range [k0, k0+w) over rows [k0, n):
+// Rows never move during the factorisation: position p lives in physical row
+// perm[p] (perm starts as the identity and the leaves exchange its entries as
+// they pivot), and every kernel reaches the rows of a position range through
+// perm. getrf_rec(k0, w) factors the column range [k0, k0+w) over positions
+// [k0, n):
 //   w <= 64: one leaf launch (leaf_kernel);
 //   else, with w1 = w/2 (a multiple of 64):
 //     getrf_rec(k0, w1)
-//     apply the left half's row moves to columns [k0+w1, k0+w)
 //     trsm_rec: A12 <- L11^-1 A12            (recursive: leaf trsm + GEMM)
 //     GEMM:     A22 <- A22 - A21 A12          (MFMA, K = w1)
 //     getrf_rec(k0+w1, w-w1)
-//     apply the right half's row moves to columns [k0, k0+w1)
 // so most flops run in GEMMs with a large K (the top level is n/2 x n/2 x n/2)
-// and the trailing matrix is read O(log n) times instead of O(n/64).
-//
-// The leaf (lu_leaf.hpp) factors a panel of up to 64 columns with one
-// workgroup per matrix and records its row moves; the row moves of a finished
-// half are composed and applied to the columns outside it in one pass.
+// and the trailing matrix is read O(log n) times instead of O(n/64). No row
+// is ever copied: a factorisation leaves logical row p of L\U in physical row
+// perm[p], which the solves read through perm; unscramble_rows gathers the
+// packed row order for callers that return the factors.
 #include <algorithm>
 #include <cstdlib>
 #include <mutex>
@@ -33,130 +34,23 @@ namespace npph {
 
 namespace {
 
-// Composes the move lists of leaves [l0, l1) (applied in order) into one net
-// list of (src, dst) row pairs over rows [klo, n). One workgroup per matrix.
-__global__ __launch_bounds__(256) void compose_kernel(int n, int klo, const int32_t* leafmv, int nleaf, int l0, int l1,
-                                                      int32_t* out, int64_t out_stride, const int32_t* mask) {
-  const int64_t mat = blockIdx.x;
-  if (mask && mask[mat] == 0) return;
-  extern __shared__ int cp[];  // n - klo entries
-  __shared__ int tmp[MVLEAF];
-  __shared__ int cnt_s;
-  const int m = n - klo, t = threadIdx.x;
-  for (int i = t; i < m; i += 256) cp[i] = i;
-  __syncthreads();
-  for (int l = l0; l < l1; ++l) {
-    const int32_t* mv = leafmv + (mat * nleaf + l) * (int64_t)LSTRIDE;
-    const int cnt = mv[0];
-    for (int e = t; e < cnt; e += 256) tmp[e] = cp[mv[1 + 2 * e] - klo];
-    __syncthreads();
-    for (int e = t; e < cnt; e += 256) cp[mv[2 + 2 * e] - klo] = tmp[e];
-    __syncthreads();
-  }
-  if (t == 0) cnt_s = 0;
-  __syncthreads();
-  int32_t* o = out + mat * out_stride;
-  for (int i = t; i < m; i += 256) {
-    if (cp[i] != i) {
-      const int e = atomicAdd(&cnt_s, 1);
-      o[1 + 2 * e] = klo + cp[i];
-      o[2 + 2 * e] = klo + i;
-    }
-  }
-  __syncthreads();
-  if (t == 0) o[0] = cnt_s;
-}
-
-// Row moves for columns [c0, c0+nc): phase 0 copies the source rows into
-// scratch, phase 1 writes them to their destinations. 256 columns x 16 moves
-// per workgroup.
-template <class T>
-__global__ __launch_bounds__(256) void move_rows_kernel(T* Abase, int64_t strideA, int n, int c0, int nc,
-                                                        const int32_t* list, int64_t list_stride, T* scratch,
-                                                        int64_t scr_stride, int phase, const int32_t* mask) {
-  const int64_t mat = blockIdx.z;
-  if (mask && mask[mat] == 0) return;
-  const int32_t* mv = list + mat * list_stride;
-  const int cnt = mv[0];
-  const int e0 = blockIdx.y * 16;
-  if (e0 >= cnt) return;
-  const int c = blockIdx.x * 256 + threadIdx.x;
-  if (c >= nc) return;
-  T* A = Abase + mat * strideA;
-  T* sc = scratch + mat * scr_stride;
-  const int e1 = min(cnt, e0 + 16);
-  T buf[16];
-  if (phase == 0) {
-#pragma unroll
-    for (int q = 0; q < 16; ++q) buf[q] = A[(int64_t)mv[1 + 2 * min(e0 + q, e1 - 1)] * n + c0 + c];
-#pragma unroll
-    for (int q = 0; q < 16; ++q)
-      if (e0 + q < e1) sc[(int64_t)(e0 + q) * nc + c] = buf[q];
-  } else {
-#pragma unroll
-    for (int q = 0; q < 16; ++q) buf[q] = sc[(int64_t)min(e0 + q, e1 - 1) * nc + c];
-#pragma unroll
-    for (int q = 0; q < 16; ++q)
-      if (e0 + q < e1) A[(int64_t)mv[2 + 2 * (e0 + q)] * n + c0 + c] = buf[q];
-  }
-}
-
-// Row moves for columns [c0, c0+nc) in one pass when at most FMV rows move:
-// a workgroup owns 16 columns, stages every moved row's values for them in
-// LDS (the move list first, so the value loads are independent), then writes
-// them to their destinations. Columns are disjoint between workgroups, so no
-// other workgroup reads what this one overwrites.
-constexpr int FMC = 16;
-template <class T>
-constexpr int fused_max_moves() { return sizeof(T) == 4 ? 512 : 256; }
-template <class T>
-__global__ __launch_bounds__(256) void move_rows_fused_kernel(T* Abase, int64_t strideA, int n, int c0, int nc,
-                                                              const int32_t* list, int64_t list_stride,
-                                                              const int32_t* mask) {
-  constexpr int FMV = fused_max_moves<T>();
-  const int64_t mat = blockIdx.y;
-  if (mask && mask[mat] == 0) return;
-  const int32_t* mv = list + mat * list_stride;
-  const int cnt = mv[0];
-  if (cnt == 0) return;
-  T* A = Abase + mat * strideA;
-  __shared__ T buf[FMV][FMC];
-  __shared__ int src[FMV], dst[FMV];
-  const int t = threadIdx.x, cl = t % FMC, g = t / FMC;
-  for (int e = t; e < cnt; e += 256) {
-    src[e] = mv[1 + 2 * e];
-    dst[e] = mv[2 + 2 * e];
-  }
-  __syncthreads();
-  const int c = blockIdx.x * FMC + cl;
-  const int cc = c0 + min(c, nc - 1);
-  constexpr int G = 256 / FMC;  // move groups
-  for (int e0 = g; e0 < cnt; e0 += 4 * G) {
-    T v[4];
-#pragma unroll
-    for (int u = 0; u < 4; ++u) v[u] = A[(int64_t)src[min(e0 + u * G, cnt - 1)] * n + cc];
-#pragma unroll
-    for (int u = 0; u < 4; ++u)
-      if (e0 + u * G < cnt) buf[e0 + u * G][cl] = v[u];
-  }
-  __syncthreads();
-  if (c < nc)
-    for (int e = g; e < cnt; e += G) A[(int64_t)dst[e] * n + c0 + c] = buf[e][cl];
-}
-
-// B <- L^-1 B for the unit-lower s x s block L at (r0, r0), s <= 64, and the
-// columns [c0, c0+nc). One thread per column (TT threads share one copy of L,
+// B <- L^-1 B for the unit-lower s x s block L at positions (r0, r0), s <= 64,
+// and the columns [c0, c0+nc) (rows through perm). One thread per column (TT threads share one copy of L,
 // stored transposed in LDS), column-oriented substitution: step p subtracts
 // L[q][p] x[p] from every later row q, so the dependent chain is one FMA per
 // step rather than a q-term dot product per row. TT = 64, 128 or 256 threads,
 // the narrowest that covers nc (idle threads would hold registers and LDS).
 template <class T, int TT>
 __global__ __launch_bounds__(TT) void trsm_leaf_kernel(T* Abase, int64_t strideA, int n, int r0, int s, int c0,
-                                                       int nc, const int32_t* mask) {
+                                                       int nc, const int32_t* permb, const int32_t* mask) {
   const int64_t mat = blockIdx.y;
   if (mask && mask[mat] == 0) return;
   T* A = Abase + mat * strideA;
   __shared__ T LT[LEAF][LEAF + 1];  // LT[p][q] = L[q][p]
+  __shared__ int rowp[LEAF];        // physical rows of positions r0 .. r0 + s - 1
+  const int32_t* perm = permb + mat * n;
+  for (int i = threadIdx.x; i < LEAF; i += TT) rowp[i] = perm[r0 + min(i, s - 1)];
+  __syncthreads();
   {
     // coalesced clamped loads of L (row i, column j), transposed into LDS
     constexpr int PER = LEAF * LEAF / TT;
@@ -164,7 +58,7 @@ __global__ __launch_bounds__(TT) void trsm_leaf_kernel(T* Abase, int64_t strideA
 #pragma unroll
     for (int u = 0; u < PER; ++u) {
       const int e = threadIdx.x + u * TT, i = e / LEAF, j = e - i * LEAF;
-      v[u] = A[(int64_t)(r0 + min(i, s - 1)) * n + r0 + min(j, s - 1)];
+      v[u] = A[(int64_t)rowp[i] * n + r0 + min(j, s - 1)];
     }
 #pragma unroll
     for (int u = 0; u < PER; ++u) {
@@ -175,10 +69,10 @@ __global__ __launch_bounds__(TT) void trsm_leaf_kernel(T* Abase, int64_t strideA
   __syncthreads();
   const int c = blockIdx.x * TT + threadIdx.x;
   if (c >= nc) return;
-  T* base = A + (int64_t)r0 * n + c0 + c;  // row q of this column at base + q * n
+  T* base = A + c0 + c;  // position r0 + q of this column at base + rowp[q] * n
   T x[LEAF];
 #pragma unroll
-  for (int q = 0; q < LEAF; ++q) x[q] = base[(int64_t)min(q, s - 1) * n];
+  for (int q = 0; q < LEAF; ++q) x[q] = base[(int64_t)rowp[q] * n];
 #pragma unroll
   for (int q = 0; q < LEAF; ++q) x[q] = q < s ? x[q] : T(0);
 #pragma unroll
@@ -194,7 +88,38 @@ __global__ __launch_bounds__(TT) void trsm_leaf_kernel(T* Abase, int64_t strideA
   }
 #pragma unroll
   for (int q = 0; q < LEAF; ++q)
-    if (q < s) base[(int64_t)q * n] = x[q];
+    if (q < s) base[(int64_t)rowp[q] * n] = x[q];
+}
+
+// Packed row order from a factorisation whose rows never moved: row p of
+// columns [c0, c0+nc) becomes physical row perm[p]'s, through scratch (phase 0
+// gathers into scratch, phase 1 writes back). 16 rows x 256 columns per
+// workgroup.
+template <class T>
+__global__ __launch_bounds__(256) void gather_rows_kernel(T* Abase, int64_t strideA, int n, int c0, int nc,
+                                                          const int32_t* permb, T* scratch, int64_t scr_stride,
+                                                          int phase) {
+  const int64_t mat = blockIdx.z;
+  const int c = blockIdx.x * 256 + threadIdx.x;
+  if (c >= nc) return;
+  const int p0 = blockIdx.y * 16;
+  T* A = Abase + mat * strideA;
+  T* sc = scratch + mat * scr_stride;
+  const int32_t* perm = permb + mat * n;
+  T buf[16];
+  if (phase == 0) {
+#pragma unroll
+    for (int q = 0; q < 16; ++q) buf[q] = A[(int64_t)perm[min(p0 + q, n - 1)] * n + c0 + c];
+#pragma unroll
+    for (int q = 0; q < 16; ++q)
+      if (p0 + q < n) sc[(int64_t)(p0 + q) * nc + c] = buf[q];
+  } else {
+#pragma unroll
+    for (int q = 0; q < 16; ++q) buf[q] = sc[(int64_t)min(p0 + q, n - 1) * nc + c];
+#pragma unroll
+    for (int q = 0; q < 16; ++q)
+      if (p0 + q < n) A[(int64_t)(p0 + q) * n + c0 + c] = buf[q];
+  }
 }
 
 __global__ void init_factor_kernel(int64_t count, int n, int32_t* perm, int32_t* info) {
@@ -225,11 +150,6 @@ struct Rec {
   const int32_t* mask;
   int bf16_terms;
   hipStream_t s;
-  int32_t* leafmv;  // count x nleaf x LSTRIDE
-  int nleaf;
-  int32_t* net;     // count x (1 + 2n)
-  T* scratch;       // count x scr_stride (row-move staging)
-  int64_t scr_stride;
 
   static int half(int w) {
     int h = (w / 2 + LEAF - 1) / LEAF * LEAF;
@@ -237,8 +157,7 @@ struct Rec {
   }
 
   int leaf(int k0, int w) {
-    LeafArgs<T> la{A, strideA, n, k0, w, ipiv, perm, info, leafmv + (int64_t)(k0 / LEAF) * LSTRIDE,
-                   (int64_t)nleaf * LSTRIDE, mask};
+    LeafArgs<T> la{A, strideA, n, k0, w, ipiv, perm, info, mask};
     const int m = n - k0;
     const size_t shm = (size_t)m * sizeof(int);
     dim3 grid((unsigned)count);
@@ -258,33 +177,10 @@ struct Rec {
     return NPPH_OK;
   }
 
-  // Apply the moves of the leaves covering steps [kl, kl+wl) to columns [c0, c0+nc).
-  int apply_moves(int kl, int wl, int c0, int nc) {
-    if (nc <= 0) return NPPH_OK;
-    const int l0 = kl / LEAF, l1 = (kl + wl - 1) / LEAF + 1;
-    const int m = n - kl;
-    hipLaunchKernelGGL(compose_kernel, dim3((unsigned)count), dim3(256), (size_t)m * sizeof(int), s, n, kl, leafmv,
-                       nleaf, l0, l1, net, (int64_t)(1 + 2 * n), mask);
-    NPPH_HIP(hipGetLastError());
-    const int maxmv = std::min(m, 2 * wl);
-    ProfScope ps(kProfMoveTrsm, 0.0, 4.0 * sizeof(T) * count * (double)maxmv * nc, s);
-    if (maxmv <= fused_max_moves<T>()) {
-      hipLaunchKernelGGL(move_rows_fused_kernel<T>, dim3((unsigned)ceil_div(nc, FMC), (unsigned)count), dim3(256), 0,
-                         s, A, strideA, n, c0, nc, net, (int64_t)(1 + 2 * n), mask);
-      NPPH_HIP(hipGetLastError());
-      return NPPH_OK;
-    }
-    if ((int64_t)maxmv * nc > scr_stride) { set_error("LU: row-move scratch too small"); return NPPH_ERR_INVALID; }
-    dim3 grid((unsigned)ceil_div(nc, 256), (unsigned)ceil_div(maxmv, 16), (unsigned)count);
-    for (int phase = 0; phase < 2; ++phase) {
-      hipLaunchKernelGGL(move_rows_kernel<T>, grid, dim3(256), 0, s, A, strideA, n, c0, nc, net, (int64_t)(1 + 2 * n),
-                         scratch, scr_stride, phase, mask);
-      NPPH_HIP(hipGetLastError());
-    }
-    return NPPH_OK;
-  }
-
-  int gemm_update(int64_t M, int64_t N, int64_t K, T* a, T* b, T* c) {
+  // C <- C - A B with the rows of A and C at positions [ra, ra+M) and the rows
+  // of B at positions [rb, rb+K) (all through perm); a, b, c point at column 0
+  // of their operand.
+  int gemm_update(int64_t M, int64_t N, int64_t K, int ra, T* a, int rb, T* b, T* c) {
     if (M <= 0 || N <= 0 || K <= 0) return NPPH_OK;
     if constexpr (sizeof(T) == 4) {
       // split-BF16 for the deep products; the shallow ones (which would pay two
@@ -292,9 +188,11 @@ struct Rec {
       // at least as accurate
       if (bf16_terms > 0 && K >= bf16_min_k() && M >= 128 && N >= 128)
         return gemm_bf16split_update(M, N, K, (const float*)a, n, strideA, (const float*)b, n, strideA, (float*)c, n,
-                                     strideA, count, bf16_terms, s);
+                                     strideA, count, bf16_terms, s,
+                                     RowMaps{perm + ra, perm + rb, perm + ra, (int64_t)n});
     }
     GemmArgs<T> g{M, N, K, T(-1), T(1), a, n, strideA, b, n, strideA, c, n, strideA, 0, 0, 0};
+    g.maps = RowMaps{perm + ra, perm + rb, perm + ra, (int64_t)n};
     return gemm_launch<T>(g, count, s);
   }
 
@@ -305,20 +203,19 @@ struct Rec {
       ProfScope ps(kProfMoveTrsm, (double)count * sz * sz * nc, 2.0 * sizeof(T) * count * (double)sz * nc, s);
       if (nc <= 64)
         hipLaunchKernelGGL((trsm_leaf_kernel<T, 64>), dim3(1u, (unsigned)count), dim3(64), 0, s, A, strideA, n, r0,
-                           sz, c0, nc, mask);
+                           sz, c0, nc, perm, mask);
       else if (nc <= 128)
         hipLaunchKernelGGL((trsm_leaf_kernel<T, 128>), dim3(1u, (unsigned)count), dim3(128), 0, s, A, strideA, n,
-                           r0, sz, c0, nc, mask);
+                           r0, sz, c0, nc, perm, mask);
       else
         hipLaunchKernelGGL((trsm_leaf_kernel<T, 256>), dim3((unsigned)ceil_div(nc, 256), (unsigned)count),
-                           dim3(256), 0, s, A, strideA, n, r0, sz, c0, nc, mask);
+                           dim3(256), 0, s, A, strideA, n, r0, sz, c0, nc, perm, mask);
       NPPH_HIP(hipGetLastError());
       return NPPH_OK;
     }
     const int s1 = half(sz);
     NPPH_TRY(trsm(r0, s1, c0, nc));
-    NPPH_TRY(gemm_update(sz - s1, nc, s1, A + (int64_t)(r0 + s1) * n + r0, A + (int64_t)r0 * n + c0,
-                         A + (int64_t)(r0 + s1) * n + c0));
+    NPPH_TRY(gemm_update(sz - s1, nc, s1, r0 + s1, A + r0, r0, A + c0, A + c0));
     return trsm(r0 + s1, sz - s1, c0, nc);
   }
 
@@ -326,29 +223,16 @@ struct Rec {
     if (w <= LEAF) return leaf(k0, w);
     const int w1 = half(w), w2 = w - w1;
     NPPH_TRY(getrf(k0, w1));
-    NPPH_TRY(apply_moves(k0, w1, k0 + w1, w2));
     NPPH_TRY(trsm(k0, w1, k0 + w1, w2));
-    NPPH_TRY(gemm_update(n - k0 - w1, w2, w1, A + (int64_t)(k0 + w1) * n + k0, A + (int64_t)k0 * n + k0 + w1,
-                         A + (int64_t)(k0 + w1) * n + k0 + w1));
-    NPPH_TRY(getrf(k0 + w1, w2));
-    return apply_moves(k0 + w1, w2, k0, w1);
+    NPPH_TRY(gemm_update(n - k0 - w1, w2, w1, k0 + w1, A + k0, k0, A + k0 + w1, A + k0 + w1));
+    return getrf(k0 + w1, w2);
   }
 };
 
 
-// Row-move staging per matrix: the largest row move is at the top level,
-// min(n, 2 w1) rows of w2 columns with w1 >= n/2, at most n * n/2 elements.
-int64_t getrf_move_stride(int n) { return (int64_t)n * (n / 2 + LEAF); }
-
-// Workspace bytes one factorisation asks for (leaf move lists, composed move
-// lists, row-move staging).
-struct GetrfWs {
-  size_t leafmv, net, move;
-};
-GetrfWs getrf_ws(int64_t count, int n, size_t elem) {
-  return {sizeof(int32_t) * (size_t)count * (size_t)ceil_div(n, LEAF) * LSTRIDE,
-          sizeof(int32_t) * (size_t)count * (1 + 2 * (size_t)n), elem * (size_t)count * getrf_move_stride(n)};
-}
+// Row-gather staging per matrix (unscramble_rows): half of the columns at a
+// time, at most n x (n/2 + 1) elements.
+int64_t gather_stride(int n) { return (int64_t)n * (n / 2 + LEAF); }
 
 template <class T>
 int getrf_group(int64_t count, int n, T* A, int64_t strideA, int32_t* ipiv, int32_t* perm, int32_t* info,
@@ -364,12 +248,6 @@ int getrf_group(int64_t count, int n, T* A, int64_t strideA, int32_t* ipiv, int3
   r.mask = mask;
   r.bf16_terms = bf16_terms;
   r.s = s;
-  r.nleaf = (int)ceil_div(n, LEAF);
-  r.scr_stride = getrf_move_stride(n);
-  const GetrfWs ws = getrf_ws(count, n, sizeof(T));
-  NPPH_TRY(workspace(kWsMisc, ws.leafmv, (void**)&r.leafmv));
-  NPPH_TRY(workspace(kWsNet, ws.net, (void**)&r.net));
-  NPPH_TRY(workspace(kWsMove, ws.move, (void**)&r.scratch));
   {
     const int64_t tot = count * n;
     hipLaunchKernelGGL(init_factor_kernel, dim3((unsigned)ceil_div(tot, 256)), dim3(256), 0, s, count, n, perm, info);
@@ -383,10 +261,30 @@ int getrf_group(int64_t count, int n, T* A, int64_t strideA, int32_t* ipiv, int3
 int max_blocked_n() { return 8 * PT; }
 
 size_t blocked_getrf_scratch_bytes(int64_t count, int n, size_t elem) {
-  const GetrfWs w = getrf_ws(count, n, elem);
-  auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
-  return al(w.leafmv) + al(w.net) + al(w.move);
+  return (elem * (size_t)count * gather_stride(n) + 255) & ~(size_t)255;
 }
+
+template <class T>
+int unscramble_rows(int64_t count, int n, T* A, int64_t strideA, const int32_t* perm, hipStream_t s) {
+  if (count <= 0 || n <= 0) return NPPH_OK;
+  if (count > 65535) { set_error("unscramble: count > 65535"); return NPPH_ERR_INVALID; }
+  T* scr;
+  NPPH_TRY(workspace(kWsMove, sizeof(T) * (size_t)count * gather_stride(n), (void**)&scr));
+  const int h = n / 2;
+  for (int part = 0; part < 2; ++part) {
+    const int c0 = part == 0 ? 0 : h, nc = part == 0 ? h : n - h;
+    if (nc <= 0) continue;
+    dim3 grid((unsigned)ceil_div(nc, 256), (unsigned)ceil_div(n, 16), (unsigned)count);
+    for (int phase = 0; phase < 2; ++phase) {
+      hipLaunchKernelGGL(gather_rows_kernel<T>, grid, dim3(256), 0, s, A, strideA, n, c0, nc, perm, scr,
+                         gather_stride(n), phase);
+      NPPH_HIP(hipGetLastError());
+    }
+  }
+  return NPPH_OK;
+}
+template int unscramble_rows<double>(int64_t, int, double*, int64_t, const int32_t*, hipStream_t);
+template int unscramble_rows<float>(int64_t, int, float*, int64_t, const int32_t*, hipStream_t);
 
 namespace {
 

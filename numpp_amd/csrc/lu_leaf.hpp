@@ -3,20 +3,21 @@
 // translation units (one per working precision and panel family) so that they
 // compile in parallel.
 //
-// The leaf factors an (n-k0) x w panel (w <= 64) with one workgroup per
-// matrix, left-looking in inner blocks of IB columns: the block's columns are
-// gathered through the leaf's row permutation (rows are not moved until the
-// block is done), brought up to date with the block's U rows (one wave per
-// column solves with L11 via v_readlane) and the L columns already computed,
-// then factored in registers with one workgroup barrier per column: each wave
-// offers its local pivot candidate together with that row's values, so every
-// thread picks the pivot (NumPP's rule: largest |a|, then smallest current
-// position, linalg.cpp:55-56, with its NaN behaviour) and reads the pivot row in
-// the same step. The per-step pivot rows (ipiv), the permutation (perm), the
-// first exact-zero pivot (info) and the leaf's net row moves for the other
-// columns are written. The rank-1 updates of linalg.cpp:64-68 are regrouped into
-// these blocks; each entry receives the same terms in a different association
-// order.
+// Rows never move: position p of the factorisation lives in physical row
+// perm[p], and pivoting only exchanges perm entries. The leaf factors the
+// (n-k0) x w panel at positions [k0, n) (w <= 64) with one workgroup per
+// matrix, left-looking in inner blocks of IB columns: each thread gathers its
+// rows' block columns from their physical rows, brings them up to date with the
+// block's U rows (one wave per column solves with L11 via v_readlane) and the L
+// columns already computed, then factors them in registers with one workgroup
+// barrier per column: each wave offers its local pivot candidate together with
+// that row's values, so every thread picks the pivot (NumPP's rule: largest
+// |a|, then smallest current position, linalg.cpp:55-56, with its NaN
+// behaviour) and reads the pivot row in the same step. The block is written
+// back to the same physical rows. The per-step pivot rows (ipiv), the
+// permutation (perm) and the first exact-zero pivot (info) are written. The
+// rank-1 updates of linalg.cpp:64-68 are regrouped into these blocks; each
+// entry receives the same terms in a different association order.
 #pragma once
 
 #include <utility>
@@ -27,8 +28,6 @@ namespace npph {
 
 constexpr int LEAF = 64;   // leaf panel width
 constexpr int PT = 512;    // leaf threads
-constexpr int MVLEAF = 2 * LEAF;  // max net moves of one leaf
-constexpr int LSTRIDE = 1 + 2 * MVLEAF;
 
 template <class T>
 struct LeafArgs {
@@ -36,10 +35,8 @@ struct LeafArgs {
   int64_t strideA;
   int n, k0, w;
   int32_t* ipiv;
-  int32_t* perm;
+  int32_t* perm;  // count x n: physical row of each position
   int32_t* info;
-  int32_t* mv;        // this leaf's list for matrix 0; matrix i at mv + i * mv_stride
-  int64_t mv_stride;
   const int32_t* mask;
 };
 
@@ -66,18 +63,15 @@ __global__ __launch_bounds__(NTH) void leaf_kernel(LeafArgs<T> a) {
   __shared__ T red_v[2][NWL];  // candidate magnitudes (exact in T)
   __shared__ int red_p[2][NWL];
   __shared__ __attribute__((aligned(16))) T red_row[2][NWL][IB];
-  __shared__ int mv_old[2 * IB], mv_new[2 * IB];
-  __shared__ int mv_cnt;
-  __shared__ T stage[2 * IB][LEFT];
-  __shared__ int stage_perm[2 * IB], stage_pid[2 * IB];
+  __shared__ int stage_row[IB];  // physical rows of the block's pivot rows
   __shared__ int sing_s;
   // pivot rows of the leaf's steps, written to HBM once after the last block:
   // a global store inside the column loop would make every column barrier
   // wait for it (the barrier's release fence)
   __shared__ int piv_s[LEAF];
-  extern __shared__ int pid[];  // m entries: leaf-start row of the row now at each position
+  extern __shared__ int rowof[];  // m entries: physical row of each leaf position
 
-  for (int i = t; i < m; i += NTH) pid[i] = i;
+  for (int i = t; i < m; i += NTH) rowof[i] = perm[k0 + i];
   if (t == 0) sing_s = a.info[mat];
   __syncthreads();
 
@@ -85,13 +79,14 @@ __global__ __launch_bounds__(NTH) void leaf_kernel(LeafArgs<T> a) {
     const int ib = FULL ? IB : min(IB, w - jb);  // FULL: w is a multiple of IB
     const int cb = k0 + jb;  // first global column of the block
     T v[R][IB];
-    int pos[R];
-    // ---- gather the block's columns (lazy: rows still at leaf-start places)
+    int pos[R], phys[R];  // current positions and (fixed) physical rows
+    // ---- gather the block's columns from the rows' physical places
 #pragma unroll
     for (int s = 0; s < R; ++s) {
       const int p0 = jb + t + NTH * s;
       pos[s] = p0 < m ? p0 : (1 << 28);
-      const T* src = A + (int64_t)(k0 + (p0 < m ? pid[p0] : 0)) * n + cb;
+      phys[s] = rowof[p0 < m ? p0 : 0];
+      const T* src = A + (int64_t)phys[s] * n + cb;
 #pragma unroll
       // the row index is clamped, so the loads need no row guard; padding
       // slots (pos >= m) are excluded from the pivot search and every write
@@ -105,7 +100,7 @@ __global__ __launch_bounds__(NTH) void leaf_kernel(LeafArgs<T> a) {
       for (int i = 0; i < UIT; ++i) {
         const int e = t + i * NTH, q = e / IB, c = e % IB;
         const bool in = e < jb * IB && c < ib;
-        u[i] = in ? A[(int64_t)(k0 + pid[in ? q : 0]) * n + cb + c] : T(0);
+        u[i] = in ? A[(int64_t)rowof[in ? q : 0] * n + cb + c] : T(0);
       }
 #pragma unroll
       for (int i = 0; i < UIT; ++i) {
@@ -125,7 +120,7 @@ __global__ __launch_bounds__(NTH) void leaf_kernel(LeafArgs<T> a) {
         }
         if (lane < jb) {
           Ublk[lane][cc] = x;
-          A[(int64_t)(k0 + lane) * n + cb + cc] = x;
+          A[(int64_t)rowof[lane] * n + cb + cc] = x;
         }
       }
     }
@@ -137,11 +132,10 @@ __global__ __launch_bounds__(NTH) void leaf_kernel(LeafArgs<T> a) {
       auto loadc = [&](T (&dst)[R][QC], int q0) {
 #pragma unroll
         for (int s = 0; s < R; ++s) {
-          const T* lr = A + (int64_t)(k0 + (pos[s] < m ? pos[s] : 0)) * n + k0 + q0;
+          const T* lr = A + (int64_t)phys[s] * n + k0 + q0;
 #pragma unroll
-          // jb is a multiple of IB and of QC, so a chunk is all inside; rows
-          // past m (clamped to row 0) only feed padding slots, which are
-          // never read back
+          // jb is a multiple of IB and of QC, so a chunk is all inside; padding
+          // slots read a real row and are never written back
           for (int q = 0; q < QC; ++q) dst[s][q] = lr[q];
         }
       };
@@ -167,9 +161,6 @@ __global__ __launch_bounds__(NTH) void leaf_kernel(LeafArgs<T> a) {
       }
     }
     // ---- factor the block's columns, one barrier per column
-    int opos[R];
-#pragma unroll
-    for (int s = 0; s < R; ++s) opos[s] = pos[s];
     // (compile-time column index: one copy of the step per column, so the
     // register rows are addressed statically)
     auto fstep = [&]<int c>() __attribute__((always_inline)) {
@@ -247,30 +238,42 @@ __global__ __launch_bounds__(NTH) void leaf_kernel(LeafArgs<T> a) {
       (fstep.template operator()<C>(), ...);
     }(std::make_integer_sequence<int, IB>{});
     // ---- L11 rows of the block's new pivot rows: block columns from
-    // registers, left columns (from their pre-block places) loaded by all
+    // registers, left columns (from the rows' physical places) loaded by all
     // threads in parallel
     // (only blocks before the last: rows and columns below LEFT)
     const bool need_l11 = jb + ib < w;
-    if (t == 0) mv_cnt = 0;
     if (need_l11) {
 #pragma unroll
       for (int s = 0; s < R; ++s) {
         if (pos[s] >= jb && pos[s] < jb + ib) {
-          stage_pid[pos[s] - jb] = opos[s];
+          stage_row[pos[s] - jb] = phys[s];
 #pragma unroll
           for (int c = 0; c < IB; ++c)
             if (c < ib) L11[pos[s]][jb + c] = v[s][c];
         }
       }
     }
-    __syncthreads();
+    // ---- the block back to its rows' physical places
+#pragma unroll
+    for (int s = 0; s < R; ++s) {
+      if (pos[s] < m) {
+        T* row = A + (int64_t)phys[s] * n + cb;
+#pragma unroll
+        for (int c = 0; c < IB; ++c)
+          if (c < ib) row[c] = v[s][c];
+      }
+    }
+    __syncthreads();  // every read of rowof for this block is done
+#pragma unroll
+    for (int s = 0; s < R; ++s)
+      if (pos[s] < m) rowof[pos[s]] = phys[s];
     if (need_l11 && jb > 0) {
       constexpr int LIT = (IB * (LEAF - IB) + NTH - 1) / NTH > 0 ? (IB * (LEAF - IB) + NTH - 1) / NTH : 1;
       T l[LIT];
 #pragma unroll
       for (int i = 0; i < LIT; ++i) {
         const int e = t + i * NTH, rr = e / jb, q = e - rr * jb;
-        l[i] = e < ib * jb ? A[(int64_t)(k0 + stage_pid[rr]) * n + k0 + q] : T(0);
+        l[i] = e < ib * jb ? A[(int64_t)stage_row[rr] * n + k0 + q] : T(0);
       }
 #pragma unroll
       for (int i = 0; i < LIT; ++i) {
@@ -279,76 +282,12 @@ __global__ __launch_bounds__(NTH) void leaf_kernel(LeafArgs<T> a) {
       }
     }
     __syncthreads();
-    // ---- write the block at final places; list rows whose place changed
-#pragma unroll
-    for (int s = 0; s < R; ++s) {
-      if (pos[s] < m) {
-        T* row = A + (int64_t)(k0 + pos[s]) * n + cb;
-#pragma unroll
-        for (int c = 0; c < IB; ++c)
-          if (c < ib) row[c] = v[s][c];
-        if (pos[s] != opos[s]) {
-          const int e = atomicAdd(&mv_cnt, 1);
-          mv_old[e] = opos[s];
-          mv_new[e] = pos[s];
-        }
-      }
-    }
-    __syncthreads();
-    // ---- move the up-to-date left columns, perm and pid of those rows
-    const int cnt = mv_cnt;
-    const int oc = jb;  // left columns [k0, cb)
-    constexpr int MIT = (2 * IB * (LEAF - IB + 2) + NTH - 1) / NTH;
-    const int tot = cnt * (oc + 2);
-    if (tot > 0) {
-      T mvv[MIT];
-      int mvi[MIT];
-#pragma unroll
-      for (int i = 0; i < MIT; ++i) {
-        const int e = t + i * NTH, q = e / (oc + 2), j = e - q * (oc + 2);
-        const int src = e < tot ? mv_old[q] : 0;
-        mvv[i] = (e < tot && j < oc) ? A[(int64_t)(k0 + src) * n + k0 + j] : T(0);
-        mvi[i] = e < tot ? (j == oc ? perm[k0 + src] : (j > oc ? pid[src] : 0)) : 0;
-      }
-#pragma unroll
-      for (int i = 0; i < MIT; ++i) {
-        const int e = t + i * NTH, q = e / (oc + 2), j = e - q * (oc + 2);
-        if (e < tot) {
-          if (j < oc) stage[q][j] = mvv[i];
-          else if (j == oc) stage_perm[q] = mvi[i];
-          else stage_pid[q] = mvi[i];
-        }
-      }
-      __syncthreads();
-#pragma unroll
-      for (int i = 0; i < MIT; ++i) {
-        const int e = t + i * NTH, q = e / (oc + 2), j = e - q * (oc + 2);
-        if (e < tot) {
-          const int dst = mv_new[q];
-          if (j < oc) A[(int64_t)(k0 + dst) * n + k0 + j] = stage[q][j];
-          else if (j == oc) perm[k0 + dst] = stage_perm[q];
-          else pid[dst] = stage_pid[q];
-        }
-      }
-    }
-    __syncthreads();
   }
 
-  // Net row moves of this leaf for the columns outside it.
-  if (t == 0) { mv_cnt = 0; a.info[mat] = sing_s; }
+  // pivots, first exact-zero pivot and the positions' physical rows
+  if (t == 0) a.info[mat] = sing_s;
   for (int c = t; c < w; c += NTH) ipiv[k0 + c] = piv_s[c];
-  __syncthreads();
-  int32_t* mv = a.mv + mat * a.mv_stride;
-  for (int i = t; i < m; i += NTH) {
-    const int src = pid[i];
-    if (src != i) {
-      const int e = atomicAdd(&mv_cnt, 1);
-      mv[1 + 2 * e] = k0 + src;
-      mv[2 + 2 * e] = k0 + i;
-    }
-  }
-  __syncthreads();
-  if (t == 0) mv[0] = mv_cnt;
+  for (int i = t; i < m; i += NTH) perm[k0 + i] = rowof[i];
 }
 
 

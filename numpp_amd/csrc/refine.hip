@@ -736,7 +736,7 @@ int eta_step(RefineRun& r, hipStream_t s) {
 int corr_step(RefineRun& r, int it, hipStream_t s) {
   if (it > max_iter_of(r.o)) return NPPH_OK;
   NPPH_TRY(getrs_refine(r.count, r.n, r.nrhs, r.f.lu32, (int64_t)r.n * r.n, r.f.perm, r.R, r.scale, r.Dlo, r.X, 0,
-                        r.active, s));
+                        r.active, s, r.f.perm));
   hipLaunchKernelGGL(count_sweep_kernel, g1(r.count), dim3(256), 0, s, r.count, r.active, r.iters);
   NPPH_HIP(hipGetLastError());
   return NPPH_OK;
@@ -786,7 +786,8 @@ int refine_begin(RefineRun& r, hipStream_t s) {
 
   if (r.o.precision == NPPH_PREC_FP64) {
     NPPH_TRY(blocked_getrs_perm<double>(count, n, nrhs, r.f.lu64, (int64_t)n * n, r.f.perm, r.B, per_b, r.X, per_b,
-                                        r.f.info, nullptr, s));
+                                        r.f.info, nullptr, s,
+                                        factor_rows_mapped(NPPH_PREC_FP64, n) ? r.f.perm : nullptr));
     hipLaunchKernelGGL(init_state_kernel, g1(count), dim3(256), 0, s, count, r.info, r.active, r.iters, nullptr);
     NPPH_TRY(report(r, s));
     r.done = true;
@@ -797,7 +798,7 @@ int refine_begin(RefineRun& r, hipStream_t s) {
   hipLaunchKernelGGL(scale_from_bmax_kernel, g1(nk), dim3(256), 0, s, nk, r.bmax, r.scale);
   // Sweep 0 (x = 0, r = b).
   NPPH_TRY(getrs_refine(count, n, nrhs, r.f.lu32, (int64_t)n * n, r.f.perm, r.B, r.scale, r.Dlo, r.X, 1, r.active,
-                        s));
+                        s, r.f.perm));
   hipLaunchKernelGGL(count_sweep_kernel, g1(count), dim3(256), 0, s, count, r.active, r.iters);
   NPPH_HIP(hipGetLastError());
   return NPPH_OK;
