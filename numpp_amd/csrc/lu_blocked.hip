@@ -171,7 +171,8 @@ struct Rec {
       const char* e = std::getenv("NUMPP_HIP_LEAF_THIN");
       return e ? std::atoi(e) : 1;
     }();
-    const bool thin = thin_env != 0 && count >= 4 * (int64_t)std::max(cu_count(), 1) && m <= 256;
+    // (NUMPP_HIP_LEAF_THIN=2: thin panels up to 256 rows for any stack size)
+    const bool thin = thin_env != 0 && (thin_env == 2 || count >= 4 * (int64_t)std::max(cu_count(), 1)) && m <= 256;
     NPPH_TRY(thin ? leaf_launch_thin(la, m, grid, shm, s) : leaf_launch_wide(la, m, grid, shm, s));
     NPPH_HIP(hipGetLastError());
     return NPPH_OK;

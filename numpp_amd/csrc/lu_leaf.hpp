@@ -64,7 +64,6 @@ __global__ __launch_bounds__(NTH) void leaf_kernel(LeafArgs<T> a) {
   __shared__ int red_p[2][NWL];
   __shared__ __attribute__((aligned(16))) T red_row[2][NWL][IB];
   __shared__ int stage_row[IB];  // physical rows of the block's pivot rows
-  __shared__ int sing_s;
   // pivot rows of the leaf's steps, written to HBM once after the last block:
   // a global store inside the column loop would make every column barrier
   // wait for it (the barrier's release fence)
@@ -72,7 +71,9 @@ __global__ __launch_bounds__(NTH) void leaf_kernel(LeafArgs<T> a) {
   extern __shared__ int rowof[];  // m entries: physical row of each leaf position
 
   for (int i = t; i < m; i += NTH) rowof[i] = perm[k0 + i];
-  if (t == 0) sing_s = a.info[mat];
+  // first exact-zero pivot step + 1 (0: none yet); every thread tracks it, as
+  // every thread sees each step's pivot value
+  int sing = a.info[mat];
   __syncthreads();
 
   for (int jb = 0; jb < w; jb += IB) {
@@ -207,12 +208,15 @@ __global__ __launch_bounds__(NTH) void leaf_kernel(LeafArgs<T> a) {
       lanes8_argmax(bv, bp);
       const int wbest = (int)__builtin_ctzll(__ballot(lane < NWL && cp == bp));
       const int p = bp;
-      if (bv == T(0) && t == 0 && sing_s == 0) sing_s = k0 + col + 1;
+      sing = (bv == T(0) && sing == 0) ? k0 + col + 1 : sing;
       if (t == 0) piv_s[col] = k0 + p;
       const T piv = red_row[buf][wbest][c];
       // FP32 factors (refined afterwards) scale by one reciprocal per column;
       // FP64 factors divide, as linalg.cpp:65
-      const T rpiv = sizeof(T) == 4 && piv != T(0) ? T(1) / piv : T(0);
+      // (FP32 factors are refined afterwards: the hardware reciprocal, within
+      // one ulp, instead of the correctly rounded division)
+      T rpiv = T(0);
+      if constexpr (sizeof(T) == 4) rpiv = piv != T(0) ? __builtin_amdgcn_rcpf(piv) : T(0);
 #pragma unroll
       for (int s = 0; s < R; ++s) {
         if (pos[s] == p) pos[s] = col;
@@ -285,7 +289,7 @@ __global__ __launch_bounds__(NTH) void leaf_kernel(LeafArgs<T> a) {
   }
 
   // pivots, first exact-zero pivot and the positions' physical rows
-  if (t == 0) a.info[mat] = sing_s;
+  if (t == 0) a.info[mat] = sing;
   for (int c = t; c < w; c += NTH) ipiv[k0 + c] = piv_s[c];
   for (int i = t; i < m; i += NTH) perm[k0 + i] = rowof[i];
 }
