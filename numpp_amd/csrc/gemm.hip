@@ -66,7 +66,9 @@ __device__ __forceinline__ void tile_coords(int bid, int tiles_m, int tiles_n, i
 // Tile BM x BN with BM, BN in {128, 64}: the waves form a 2 x 2 grid of
 // (BM/2) x (BN/2) sub-tiles (64-wide tiles serve narrow or short products, where
 // a 128-wide tile would leave MFMA rows or columns empty).
-template <class T, int BM, int BN>
+// MAP: rows reached through p.maps (the recursive LU's unmoved rows); without
+// it the kernel is the plain strided GEMM and keeps its registers for tiles.
+template <class T, int BM, int BN, bool MAP>
 __global__ __launch_bounds__(NT, 2) void gemm_kernel(GemmArgs<T> p) {
   typedef typename Mfma<T>::acc_t acc_t;
   constexpr int IT = BM / 32;      // 16-row MFMA tiles per wave
@@ -92,23 +94,29 @@ __global__ __launch_bounds__(NT, 2) void gemm_kernel(GemmArgs<T> p) {
   const int b_k = tid >> 4, b_col = (tid & 15) * BE;  // B tile: BK rows x BN
   const bool mn_full = (m0 + BM <= p.M) && (n0 + BN <= p.N);
   // physical rows (row maps): this thread's A row, and B rows per K slice
-  const int32_t* mapb = p.maps.b ? p.maps.b + bz * p.maps.stride : nullptr;
-  const int64_t a_phys = p.maps.a ? p.maps.a[bz * p.maps.stride + min(m0 + a_row, p.M - 1)] : m0 + a_row;
+  const int32_t* mapb = MAP && p.maps.b ? p.maps.b + bz * p.maps.stride : nullptr;
+  const int64_t a_phys =
+      MAP && p.maps.a ? (int64_t)p.maps.a[bz * p.maps.stride + min(m0 + a_row, p.M - 1)] : m0 + a_row;
   const int lk = lane >> 4, lr = lane & 15;
   // the epilogue's C rows, fetched now with the other map entries so that the
   // kernel pays one map latency, not one at each end
-  int32_t c_phys[IT][4];
+  int32_t c_phys[MAP ? IT : 1][4];
+  if constexpr (MAP) {
 #pragma unroll
-  for (int i = 0; i < IT; ++i)
+    for (int i = 0; i < IT; ++i)
 #pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const int64_t row = min(m0 + wm * (BM / 2) + i * 16 + Mfma<T>::row(lk, v), p.M - 1);
-      c_phys[i][v] = p.maps.c ? p.maps.c[bz * p.maps.stride + row] : (int32_t)row;
-    }
+      for (int v = 0; v < 4; ++v) {
+        const int64_t row = min(m0 + wm * (BM / 2) + i * 16 + Mfma<T>::row(lk, v), p.M - 1);
+        c_phys[i][v] = p.maps.c ? p.maps.c[bz * p.maps.stride + row] : (int32_t)row;
+      }
+  }
 
   // B's physical row for the slice starting at k0 (a map entry is fetched two
   // slices ahead of its use, so its latency is not added to the tile loads)
-  auto brow = [&](int64_t k0) -> int64_t { return mapb ? mapb[min(k0 + b_k, p.K - 1)] : k0 + b_k; };
+  auto brow = [&](int64_t k0) -> int64_t {
+    if constexpr (MAP) return mapb ? mapb[min(k0 + b_k, p.K - 1)] : k0 + b_k;
+    return k0 + b_k;
+  };
   T ra[AE], rb[BE];
   auto load = [&](int64_t k0, int64_t b_phys) {
     const bool full = mn_full && (k0 + BK <= p.K) && p.vec;
@@ -198,7 +206,7 @@ __global__ __launch_bounds__(NT, 2) void gemm_kernel(GemmArgs<T> p) {
     for (int v = 0; v < 4; ++v) {
       const int64_t row = m0 + wm * (BM / 2) + i * 16 + Mfma<T>::row(lk, v);
       if (row >= p.M) continue;
-      T* crow = C + (int64_t)c_phys[i][v] * p.ldc;
+      T* crow = C + (MAP ? (int64_t)c_phys[MAP ? i : 0][v] : row) * p.ldc;
 #pragma unroll
       for (int j = 0; j < JT; ++j) {
         const int64_t col = n0 + wn * (BN / 2) + j * 16 + lr;
@@ -262,10 +270,18 @@ int gemm_launch(GemmArgs<T> p, int64_t batch, hipStream_t s, int prof) {
   const double by = es * batch * ((double)p.M * p.K + (double)p.K * p.N + (p.beta == T(0) ? 1.0 : 2.0) * p.M * p.N);
   const int kind = prof == -1 ? (sizeof(T) == 8 ? kProfGemmF64 : kProfGemmF32) : prof;
   ProfScope ps(kind, fl, by, s);
-  if (shortm && narrow) hipLaunchKernelGGL((gemm_kernel<T, 64, 64>), grid, dim3(NT), 0, s, p);
-  else if (shortm) hipLaunchKernelGGL((gemm_kernel<T, 64, 128>), grid, dim3(NT), 0, s, p);
-  else if (narrow) hipLaunchKernelGGL((gemm_kernel<T, 128, 64>), grid, dim3(NT), 0, s, p);
-  else hipLaunchKernelGGL((gemm_kernel<T, 128, 128>), grid, dim3(NT), 0, s, p);
+  const bool mapped = p.maps.a || p.maps.b || p.maps.c;
+  if (mapped) {
+    if (shortm && narrow) hipLaunchKernelGGL((gemm_kernel<T, 64, 64, true>), grid, dim3(NT), 0, s, p);
+    else if (shortm) hipLaunchKernelGGL((gemm_kernel<T, 64, 128, true>), grid, dim3(NT), 0, s, p);
+    else if (narrow) hipLaunchKernelGGL((gemm_kernel<T, 128, 64, true>), grid, dim3(NT), 0, s, p);
+    else hipLaunchKernelGGL((gemm_kernel<T, 128, 128, true>), grid, dim3(NT), 0, s, p);
+  } else {
+    if (shortm && narrow) hipLaunchKernelGGL((gemm_kernel<T, 64, 64, false>), grid, dim3(NT), 0, s, p);
+    else if (shortm) hipLaunchKernelGGL((gemm_kernel<T, 64, 128, false>), grid, dim3(NT), 0, s, p);
+    else if (narrow) hipLaunchKernelGGL((gemm_kernel<T, 128, 64, false>), grid, dim3(NT), 0, s, p);
+    else hipLaunchKernelGGL((gemm_kernel<T, 128, 128, false>), grid, dim3(NT), 0, s, p);
+  }
   NPPH_HIP(hipGetLastError());
   return NPPH_OK;
 }

@@ -68,10 +68,19 @@ __global__ __launch_bounds__(NT) void sweep_kernel(int n, int nrhs, const T* LUb
   const int c0 = (int)blockIdx.y * CW, nc = min(CW, nrhs - c0);
   const T* __restrict__ LU = LUb + mat * strideA;
   T* __restrict__ X = Xb + mat * strideX;
-  // factor row of position p (factors whose rows never moved carry a row map)
+  // factor row of position p (factors whose rows never moved carry a row map;
+  // orders up to 256 keep it in LDS, the sweeps' chains read it at every block)
   const int32_t* rowmap = rowmapb ? rowmapb + mat * n : nullptr;
-  auto rowp = [&](int p) -> int64_t { return rowmap ? rowmap[p] : p; };
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  __shared__ int rmap_s[NT == 256 ? 256 : 1];
+  if constexpr (NT == 256) {
+    if (rowmap)
+      for (int i = t; i < n; i += NT) rmap_s[i] = rowmap[i];
+  }
+  auto rowp = [&](int p) -> int64_t {
+    if constexpr (NT == 256) return rowmap ? rmap_s[p] : p;
+    return rowmap ? rowmap[p] : p;
+  };
   extern __shared__ unsigned char smem_raw[];
   T* xs = reinterpret_cast<T*>(smem_raw);  // n x CW
   __shared__ T part[GB][CW];
