@@ -407,7 +407,7 @@ def bench_plain64(args):
     lib.npph_device_sync()
     lib.npph_prof_enable(0)
     prof = prof_report(lib, ctypes)
-    roof = roofline_from(prof, "lu_wave64")
+    roof = roofline_from(prof, "lu_wave64", traffic_for("lu_wave64", "plain64"))
     x = dB.to_host()
     xo, _ = oracle.batched_solve(a, b)
     reps = 20
