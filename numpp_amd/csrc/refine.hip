@@ -59,8 +59,10 @@ __global__ __launch_bounds__(256) void demote_kernel(int64_t count, int n, const
 // (nrhs rounded up to a power of two, at most 64) by 256 / CPW rows per pass,
 // reduced across a wave's row groups with xor shuffles and across the four
 // waves in LDS; every maximum is written once (no atomics).
-__global__ __launch_bounds__(256) void bnorm_kernel(int n, int nrhs, const double* Bb, double* bmax) {
+// (ctr, when given: the run's two counters, zeroed by workgroup 0.)
+__global__ __launch_bounds__(256) void bnorm_kernel(int n, int nrhs, const double* Bb, double* bmax, int32_t* ctr) {
   const int64_t mat = blockIdx.x;
+  if (ctr && mat == 0 && threadIdx.x < 2) ctr[threadIdx.x] = 0;
   const double* __restrict__ B = Bb + mat * (int64_t)n * nrhs;
   __shared__ double red[4][64];
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
@@ -115,24 +117,53 @@ __global__ void fill_kernel(double* p, int64_t n, double v) {
   if (i < n) p[i] = v;
 }
 
-// n_stop counts the systems set to 2 (stopped without convergence).
-__global__ void init_state_kernel(int64_t count, const int32_t* info, int32_t* active, int32_t* iters,
-                                  int32_t* n_stop) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= count) return;
-  active[i] = info[i] == 0 ? 1 : 2;  // a low-precision zero pivot goes to the FP64 fallback
-  if (info[i] != 0 && n_stop) atomicAdd(n_stop, 1);
-  iters[i] = 0;
-}
-
 __device__ __forceinline__ double pow2_scale(double m) {
   // 2^-e with m in [2^e, 2^(e+1)); 1 for zero / non-finite.
   if (!(m > 0.0) || !isfinite(m)) return 1.0;
   return ldexp(1.0, -ilogb(m));
 }
 
-__global__ void count_sweep_kernel(int64_t count, const int32_t* active, int32_t* iters) {
+// Adds the wave's count of `flag` to *ctr with one atomic (every lane of the
+// wave must call it).
+__device__ __forceinline__ void wave_count(bool flag, int32_t* ctr) {
+  const uint64_t m = __ballot(flag);
+  if (ctr && m && (threadIdx.x & 63) == 0) atomicAdd(ctr, (int32_t)__popcll(m));
+}
+
+// State of a refined solve after bnorm_kernel (one thread per system): the
+// column maxima of r and x cleared, berr = NaN, the first sweep's residual
+// scales from ||b||, prev = +inf, info from the factorisation, active = 1 (a
+// low-precision zero pivot: 2, for the FP64 fallback, counted in n_stop) and
+// iters = 0.
+__global__ void begin_state_kernel(int64_t count, int nrhs, const double* bmax, double* rmax, double* xmax,
+                                   double* scale, double* berr, double* prev, const int32_t* finfo, int32_t* info,
+                                   int32_t* active, int32_t* iters, int32_t* n_stop) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  bool stop = false;
+  if (i < count) {
+    for (int c = 0; c < nrhs; ++c) {
+      const int64_t k = i * nrhs + c;
+      rmax[k] = 0.0;
+      xmax[k] = 0.0;
+      berr[k] = NAN;
+      scale[k] = pow2_scale(bmax[k]);
+    }
+    prev[i] = INFINITY;
+    const int32_t f = finfo[i];
+    info[i] = f;
+    active[i] = f == 0 ? 1 : 2;
+    iters[i] = 0;
+    stop = f != 0;
+  }
+  wave_count(stop, n_stop);
+}
+
+// One more sweep for the systems still iterating; ctr (when given) is the
+// active-system counter of the next test, cleared here (its last value has
+// already been copied out on this stream).
+__global__ void count_sweep_kernel(int64_t count, const int32_t* active, int32_t* iters, int32_t* ctr) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (ctr && i == 0) *ctr = 0;
   if (i < count && active[i] == 1) iters[i] += 1;
 }
 
@@ -246,18 +277,33 @@ __global__ __launch_bounds__(256) void residual_partial_kernel(int n, int nrhs, 
   }
 }
 
+// One thread per element. When nrhs divides 64 (and so the per-system
+// element count), a wave covers whole rows of one system: its lanes of one
+// column first reduce |r| with xor shuffles, so each column takes one atomic
+// per wave instead of one per element.
 __global__ void residual_finish_kernel(int64_t count, int n, int nrhs, int nchunk, const double* part,
                                        const double* Bb, double* Rb, double* rmax, const int32_t* active) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t per = (int64_t)n * nrhs;
-  if (i >= count * per) return;
   const int64_t mat = i / per, e = i - mat * per;
-  if (active && active[mat] != 1) return;
-  double s = 0.0;
-  for (int ch = 0; ch < nchunk; ++ch) s += part[(mat * nchunk + ch) * per + e];
-  const double res = Bb[i] - s;
-  Rb[i] = res;
-  atomic_max_nonneg(rmax + mat * nrhs + (int)(e % nrhs), fabs(res));
+  const bool live = i < count * per && !(active && active[mat] != 1);
+  double m = 0.0;
+  if (live) {
+    double s = 0.0;
+    for (int ch = 0; ch < nchunk; ++ch) s += part[(mat * nchunk + ch) * per + e];
+    const double res = Bb[i] - s;
+    Rb[i] = res;
+    m = fabs(res);
+  }
+  const int lane = threadIdx.x & 63;
+  if (64 % nrhs == 0 && per % 64 == 0) {
+    // the wave is one system's (live is uniform over it); lanes l and
+    // l + k * nrhs share a column
+    for (int o = nrhs; o < 64; o <<= 1) m = fmax(m, __shfl_xor(m, o, 64));
+    if (live && lane < nrhs) atomic_max_nonneg(rmax + mat * nrhs + (int)(e % nrhs), m);
+  } else if (live) {
+    atomic_max_nonneg(rmax + mat * nrhs + (int)(e % nrhs), m);
+  }
 }
 
 // R = B - A X with the per-column maxima of |r| and |x|, for n <= 256: one
@@ -496,33 +542,34 @@ __global__ void eta_kernel(int64_t count, int nrhs, double tol, int max_iter, co
                            double* berr, int32_t* active, const int32_t* iters, int32_t* n_active,
                            int32_t* n_stop) {
   const int64_t mat = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (mat >= count || active[mat] != 1) return;
-  double worst = 0.0;
-  for (int c = 0; c < nrhs; ++c) {
-    const int64_t k = mat * nrhs + c;
-    const double rm = rmax[k], xm = xmax[k];
-    const double den = anorm[mat] * xm + bmax[k];
-    const double eta = den > 0.0 ? rm / den : (rm > 0.0 ? INFINITY : 0.0);
-    if (berr) berr[k] = eta;
-    worst = (eta > worst || isnan(eta)) ? eta : worst;
-    scale[k] = pow2_scale(rm);
-    rmax[k] = 0.0;
-    xmax[k] = 0.0;
+  int state = 0;  // 1: iterates on, 2: stopped here
+  if (mat < count && active[mat] == 1) {
+    double worst = 0.0;
+    for (int c = 0; c < nrhs; ++c) {
+      const int64_t k = mat * nrhs + c;
+      const double rm = rmax[k], xm = xmax[k];
+      const double den = anorm[mat] * xm + bmax[k];
+      const double eta = den > 0.0 ? rm / den : (rm > 0.0 ? INFINITY : 0.0);
+      if (berr) berr[k] = eta;
+      worst = (eta > worst || isnan(eta)) ? eta : worst;
+      scale[k] = pow2_scale(rm);
+      rmax[k] = 0.0;
+      xmax[k] = 0.0;
+    }
+    if (worst <= tol) {
+      active[mat] = 0;
+    } else if (iters[mat] >= max_iter || !(worst < 0.5 * prev[mat])) {
+      active[mat] = 2;
+      state = 2;
+    } else {
+      state = 1;
+    }
+    prev[mat] = worst;
   }
-  if (worst <= tol) {
-    active[mat] = 0;
-  } else if (iters[mat] >= max_iter || !(worst < 0.5 * prev[mat])) {
-    active[mat] = 2;
-    if (n_stop) atomicAdd(n_stop, 1);
-  } else {
-    atomicAdd(n_active, 1);
-  }
-  prev[mat] = worst;
-}
-
-__global__ void scale_from_bmax_kernel(int64_t n, const double* bmax, double* scale) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) scale[i] = pow2_scale(bmax[i]);
+  // one atomic per wave and counter (thousands of systems would otherwise
+  // serialise on the two addresses)
+  wave_count(state == 1, n_active);
+  wave_count(state == 2, n_stop);
 }
 
 // Systems that did not converge (active == 2): mask for the FP64 fallback.
@@ -724,7 +771,6 @@ int max_iter_of(const npph_refine_opts& o) { return o.max_iter > 0 ? o.max_iter 
 // stopped without convergence.
 int eta_step(RefineRun& r, hipStream_t s) {
   NPPH_TRY(residual_run(r, s));
-  hipLaunchKernelGGL(zero_int_kernel, dim3(1), dim3(64), 0, s, r.ctr, 1);
   hipLaunchKernelGGL(eta_kernel, g1(r.count), dim3(256), 0, s, r.count, r.nrhs, r.o.tol, max_iter_of(r.o), r.f.anorm,
                      r.bmax, r.rmax, r.xmax, r.scale, r.prev, r.berr_p, r.active, r.iters, r.ctr, r.ctr + 1);
   NPPH_HIP(hipGetLastError());
@@ -734,10 +780,13 @@ int eta_step(RefineRun& r, hipStream_t s) {
 // The correction of loop iteration `it` for the systems still iterating
 // (none past max_iter).
 int corr_step(RefineRun& r, int it, hipStream_t s) {
-  if (it > max_iter_of(r.o)) return NPPH_OK;
+  if (it > max_iter_of(r.o)) {
+    hipLaunchKernelGGL(zero_int_kernel, dim3(1), dim3(64), 0, s, r.ctr, 1);  // the next test's counter
+    return NPPH_OK;
+  }
   NPPH_TRY(getrs_refine(r.count, r.n, r.nrhs, r.f.lu32, (int64_t)r.n * r.n, r.f.perm, r.R, r.scale, r.Dlo, r.X, 0,
                         r.active, s, r.f.perm));
-  hipLaunchKernelGGL(count_sweep_kernel, g1(r.count), dim3(256), 0, s, r.count, r.active, r.iters);
+  hipLaunchKernelGGL(count_sweep_kernel, g1(r.count), dim3(256), 0, s, r.count, r.active, r.iters, r.ctr);
   NPPH_HIP(hipGetLastError());
   return NPPH_OK;
 }
@@ -776,30 +825,25 @@ int refine_begin(RefineRun& r, hipStream_t s) {
   r.berr_p = r.berr_out ? r.berr_out : berr_ws;
   NPPH_TRY(workspace(kWsR, sizeof(double) * (size_t)nb_all, (void**)&r.R));
 
-  hipLaunchKernelGGL(fill_kernel, g1(3 * nk), dim3(256), 0, s, r.bmax, (int64_t)(3 * nk), 0.0);
-  hipLaunchKernelGGL(fill_kernel, g1(count), dim3(256), 0, s, r.prev, count, (double)INFINITY);
-  hipLaunchKernelGGL(fill_kernel, g1(nk), dim3(256), 0, s, r.berr_p, nk, (double)NAN);
-  hipLaunchKernelGGL(zero_int_kernel, dim3(1), dim3(64), 0, s, r.ctr, 2);
-  hipLaunchKernelGGL(bnorm_kernel, dim3((unsigned)count), dim3(256), 0, s, n, nrhs, r.B, r.bmax);
-  NPPH_HIP(hipMemcpyAsync(r.info, r.f.info, sizeof(int32_t) * count, hipMemcpyDeviceToDevice, s));
+  hipLaunchKernelGGL(bnorm_kernel, dim3((unsigned)count), dim3(256), 0, s, n, nrhs, r.B, r.bmax, r.ctr);
+  const bool fp64 = r.o.precision == NPPH_PREC_FP64;
+  hipLaunchKernelGGL(begin_state_kernel, g1(count), dim3(256), 0, s, count, nrhs, r.bmax, r.rmax, r.xmax, r.scale,
+                     r.berr_p, r.prev, r.f.info, r.info, r.active, r.iters, fp64 ? nullptr : r.ctr + 1);
   NPPH_HIP(hipGetLastError());
 
   if (r.o.precision == NPPH_PREC_FP64) {
     NPPH_TRY(blocked_getrs_perm<double>(count, n, nrhs, r.f.lu64, (int64_t)n * n, r.f.perm, r.B, per_b, r.X, per_b,
                                         r.f.info, nullptr, s,
                                         factor_rows_mapped(NPPH_PREC_FP64, n) ? r.f.perm : nullptr));
-    hipLaunchKernelGGL(init_state_kernel, g1(count), dim3(256), 0, s, count, r.info, r.active, r.iters, nullptr);
     NPPH_TRY(report(r, s));
     r.done = true;
     return NPPH_OK;
   }
   NPPH_TRY(workspace(kWsD, sizeof(float) * (size_t)nb_all, (void**)&r.Dlo));
-  hipLaunchKernelGGL(init_state_kernel, g1(count), dim3(256), 0, s, count, r.info, r.active, r.iters, r.ctr + 1);
-  hipLaunchKernelGGL(scale_from_bmax_kernel, g1(nk), dim3(256), 0, s, nk, r.bmax, r.scale);
   // Sweep 0 (x = 0, r = b).
   NPPH_TRY(getrs_refine(count, n, nrhs, r.f.lu32, (int64_t)n * n, r.f.perm, r.B, r.scale, r.Dlo, r.X, 1, r.active,
                         s, r.f.perm));
-  hipLaunchKernelGGL(count_sweep_kernel, g1(count), dim3(256), 0, s, count, r.active, r.iters);
+  hipLaunchKernelGGL(count_sweep_kernel, g1(count), dim3(256), 0, s, count, r.active, r.iters, r.ctr);
   NPPH_HIP(hipGetLastError());
   return NPPH_OK;
 }
