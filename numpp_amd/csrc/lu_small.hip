@@ -35,7 +35,6 @@ namespace npph {
 
 namespace {
 
-constexpr int NR = 8;  // right-hand-side columns held in registers per pass
 
 template <int N>
 struct Small {
@@ -116,7 +115,9 @@ __device__ __forceinline__ void factor(SmallShared<N>& sh, double (&a)[Small<N>:
   }(std::make_integer_sequence<int, N>{});
 }
 
-template <int N, int W>
+// NR: right-hand-side columns held in registers per pass (1 for a single
+// right-hand side, else 8).
+template <int N, int W, int NR>
 __device__ __forceinline__ void wave_body(const UniformBatch& u, SmallShared<N>& sh, int64_t mat, int lane) {
   constexpr int CPW = Small<N>::CPW, NW = Small<N>::NW;
   const int n = u.n;
@@ -211,7 +212,7 @@ __device__ __forceinline__ void wave_body(const UniformBatch& u, SmallShared<N>&
 }
 
 // Uniform stack: one workgroup of NW waves per matrix.
-template <int N>
+template <int N, int NR>
 __global__ __launch_bounds__(Small<N>::NT) void lu_small_kernel(UniformBatch u) {
   __shared__ SmallShared<N> sh;
   const int64_t mat = blockIdx.x;
@@ -220,19 +221,9 @@ __global__ __launch_bounds__(Small<N>::NT) void lu_small_kernel(UniformBatch u) 
   const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   if (threadIdx.x == 0) sh.sing = -1;
   __syncthreads();
-  if constexpr (Small<N>::NW == 1) {
-    wave_body<N, 0>(u, sh, mat, lane);
-  } else if constexpr (Small<N>::NW == 2) {
-    if (wave == 0) wave_body<N, 0>(u, sh, mat, lane);
-    else wave_body<N, 1>(u, sh, mat, lane);
-  } else {
-    switch (wave) {
-      case 0: wave_body<N, 0>(u, sh, mat, lane); break;
-      case 1: wave_body<N, 1>(u, sh, mat, lane); break;
-      case 2: wave_body<N, 2>(u, sh, mat, lane); break;
-      default: wave_body<N, 3>(u, sh, mat, lane); break;
-    }
-  }
+  [&]<int... W>(std::integer_sequence<int, W...>) __attribute__((always_inline)) {
+    ((wave == W ? (wave_body<N, W, NR>(u, sh, mat, lane), 0) : 0), ...);
+  }(std::make_integer_sequence<int, Small<N>::NW>{});
 }
 
 }  // namespace
@@ -248,10 +239,16 @@ int small_lu_launch(const UniformBatch& u, hipStream_t st) {
   const double n = u.n, flu = n * (n - 1) / 2 + (n - 1) * n * (2 * n - 1) / 3;
   ProfScope ps(kProfSmallLu, u.count * (flu + (2 * n * n - n) * u.nrhs),
                8.0 * u.count * (n * n + 2 * n * u.nrhs + (u.LU ? n * n : 0)), st);
-  if (u.n <= 8) hipLaunchKernelGGL(lu_small_kernel<8>, grid, dim3(Small<8>::NT), 0, st, u);
-  else if (u.n <= 16) hipLaunchKernelGGL(lu_small_kernel<16>, grid, dim3(Small<16>::NT), 0, st, u);
-  else if (u.n <= 32) hipLaunchKernelGGL(lu_small_kernel<32>, grid, dim3(Small<32>::NT), 0, st, u);
-  else if (u.n <= 64) hipLaunchKernelGGL(lu_small_kernel<64>, grid, dim3(Small<64>::NT), 0, st, u);
+  // a single right-hand side (or none) solves one register column per pass
+  const bool one = u.nrhs <= 1;
+auto launch = [&]<int NV>() {
+    if (one) hipLaunchKernelGGL((lu_small_kernel<NV, 1>), grid, dim3(Small<NV>::NT), 0, st, u);
+    else hipLaunchKernelGGL((lu_small_kernel<NV, 8>), grid, dim3(Small<NV>::NT), 0, st, u);
+  };
+  if (u.n <= 8) launch.template operator()<8>();
+  else if (u.n <= 16) launch.template operator()<16>();
+  else if (u.n <= 32) launch.template operator()<32>();
+  else if (u.n <= 64) launch.template operator()<64>();
   else { set_error("small_lu: n=%d > 64", u.n); return NPPH_ERR_INVALID; }
   NPPH_HIP(hipGetLastError());
   return NPPH_OK;
