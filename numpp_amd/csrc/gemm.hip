@@ -9,9 +9,12 @@
 // each wave owns a 64 x 64 sub-tile (4 x 4 MFMA tiles, 16 accumulators).
 // K is staged in BK = 16 slices through double-buffered LDS: A is stored
 // k-major (transposed on the LDS write) so that both operands are read as 16
-// consecutive elements per k row, which keeps the MFMA operand reads
-// conflict-free. Global loads for slice t+1 are issued before the MFMAs of
-// slice t so their latency hides behind the matrix core.
+// consecutive elements per k row. The rows are padded to BM + 16 elements, so
+// the four k rows one MFMA operand read touches fall in disjoint banks, and the
+// loads are laid out so that the LDS writes are bank-free too: consecutive
+// lanes take consecutive A rows (one element each per transposed write) and
+// consecutive 16-byte B chunks. Global loads for slice t+1 are issued before
+// the MFMAs of slice t so their latency hides behind the matrix core.
 //
 // Tile order: blockIdx.x is remapped so that consecutive tiles land on the same
 // XCD (workgroups are dispatched round-robin over the 8 XCDs, each with its own
@@ -28,7 +31,7 @@ namespace npph {
 namespace {
 
 constexpr int BK = 16, NT = 256;
-constexpr int PAD = 4;
+constexpr int PAD = 16;
 
 template <class T>
 struct Mfma;
@@ -89,9 +92,15 @@ __global__ __launch_bounds__(NT, 2) void gemm_kernel(GemmArgs<T> p) {
   const int lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
 
-  // Global-load mapping: AE (BE) consecutive elements per thread and operand.
-  const int a_row = tid / (BK / AE), a_k = (tid % (BK / AE)) * AE;  // A tile: BM rows x BK
-  const int b_k = tid >> 4, b_col = (tid & 15) * BE;  // B tile: BK rows x BN
+  // Global-load mapping. A tile (BM rows x BK): AE consecutive k of one row per
+  // thread, consecutive lanes on consecutive rows. B tile (BK rows x BN): 16
+  // threads per row, each taking BE / VW chunks of VW elements 16 chunks apart.
+  constexpr int VW = 16 / sizeof(T);  // elements per 16-byte vector
+  constexpr int BV = BE / VW;         // B vectors per thread per slice
+  static_assert((BK / AE) * BM == NT && BK * 16 == NT && BE % VW == 0, "tile mapping");
+  const int a_row = tid % BM, a_k = (tid / BM) * AE;
+  const int b_k = tid >> 4, b_c = tid & 15;
+  auto bcol = [&](int i) { return (b_c + 16 * (i / VW)) * VW + i % VW; };
   const bool mn_full = (m0 + BM <= p.M) && (n0 + BN <= p.N);
   // physical rows (row maps): this thread's A row, and B rows per K slice
   const int32_t* mapb = MAP && p.maps.b ? p.maps.b + bz * p.maps.stride : nullptr;
@@ -121,10 +130,10 @@ __global__ __launch_bounds__(NT, 2) void gemm_kernel(GemmArgs<T> p) {
   auto load = [&](int64_t k0, int64_t b_phys) {
     const bool full = mn_full && (k0 + BK <= p.K) && p.vec;
     const int64_t gr = m0 + a_row, gk = k0 + a_k;
-    const int64_t bk = k0 + b_k, bc = n0 + b_col;
+    const int64_t bk = k0 + b_k;
     if (full) {
       const T* pa = A + a_phys * p.lda + gk;
-      const T* pb = B + b_phys * p.ldb + bc;
+      const T* pb = B + b_phys * p.ldb + n0 + b_c * VW;
       if constexpr (sizeof(T) == 8) {
 #pragma unroll
         for (int i = 0; i < AE; i += 2) {
@@ -132,9 +141,9 @@ __global__ __launch_bounds__(NT, 2) void gemm_kernel(GemmArgs<T> p) {
           ra[i] = va.x; ra[i + 1] = va.y;
         }
 #pragma unroll
-        for (int i = 0; i < BE; i += 2) {
-          double2 vb = *reinterpret_cast<const double2*>(pb + i);
-          rb[i] = vb.x; rb[i + 1] = vb.y;
+        for (int q = 0; q < BV; ++q) {
+          double2 vb = *reinterpret_cast<const double2*>(pb + 16 * VW * q);
+          rb[2 * q] = vb.x; rb[2 * q + 1] = vb.y;
         }
       } else {
 #pragma unroll
@@ -143,23 +152,26 @@ __global__ __launch_bounds__(NT, 2) void gemm_kernel(GemmArgs<T> p) {
           ra[i] = va.x; ra[i + 1] = va.y; ra[i + 2] = va.z; ra[i + 3] = va.w;
         }
 #pragma unroll
-        for (int i = 0; i < BE; i += 4) {
-          float4 vb = *reinterpret_cast<const float4*>(pb + i);
-          rb[i] = vb.x; rb[i + 1] = vb.y; rb[i + 2] = vb.z; rb[i + 3] = vb.w;
+        for (int q = 0; q < BV; ++q) {
+          float4 vb = *reinterpret_cast<const float4*>(pb + 16 * VW * q);
+          rb[4 * q] = vb.x; rb[4 * q + 1] = vb.y; rb[4 * q + 2] = vb.z; rb[4 * q + 3] = vb.w;
         }
       }
     } else {
 #pragma unroll
       for (int i = 0; i < AE; ++i) ra[i] = (gr < p.M && gk + i < p.K) ? A[a_phys * p.lda + gk + i] : T(0);
 #pragma unroll
-      for (int i = 0; i < BE; ++i) rb[i] = (bk < p.K && bc + i < p.N) ? B[b_phys * p.ldb + bc + i] : T(0);
+      for (int i = 0; i < BE; ++i) {
+        const int64_t bc = n0 + bcol(i);
+        rb[i] = (bk < p.K && bc < p.N) ? B[b_phys * p.ldb + bc] : T(0);
+      }
     }
   };
   auto store = [&](int buf) {
 #pragma unroll
     for (int i = 0; i < AE; ++i) As[buf][a_k + i][a_row] = ra[i];
 #pragma unroll
-    for (int i = 0; i < BE; ++i) Bs[buf][b_k][b_col + i] = rb[i];
+    for (int i = 0; i < BE; ++i) Bs[buf][b_k][bcol(i)] = rb[i];
   };
 
   acc_t acc[IT][JT];
