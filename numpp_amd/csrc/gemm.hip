@@ -180,9 +180,31 @@ __global__ __launch_bounds__(NT, 2) void gemm_kernel(GemmArgs<T> p) {
 #pragma unroll
     for (int j = 0; j < JT; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
 
+  // The 64 x 64 mapped tile (the recursive LU's short updates, a few K slices
+  // each) fetches its C values with the first slice, so their latency overlaps
+  // the product instead of following it.
+  constexpr bool PC = MAP && BM == 64 && BN == 64;
+  const bool beta0 = (p.beta == T(0));
+  T cpre[PC ? IT : 1][4][PC ? JT : 1];
   const int kt_total = (int)((p.K + BK - 1) / BK);
   int64_t bnext = brow(BK);
   load(0, brow(0));
+  if constexpr (PC) {
+    if (!beta0) {
+#pragma unroll
+      for (int i = 0; i < IT; ++i)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int64_t row = m0 + wm * (BM / 2) + i * 16 + Mfma<T>::row(lk, v);
+          const T* crow = C + (int64_t)c_phys[i][v] * p.ldc;
+#pragma unroll
+          for (int j = 0; j < JT; ++j) {
+            const int64_t col = n0 + wn * (BN / 2) + j * 16 + lr;
+            cpre[i][v][j] = (row < p.M && col < p.N) ? crow[col] : T(0);
+          }
+        }
+    }
+  }
   store(0);
   __syncthreads();
   for (int kt = 0; kt < kt_total; ++kt) {
@@ -211,7 +233,6 @@ __global__ __launch_bounds__(NT, 2) void gemm_kernel(GemmArgs<T> p) {
   }
 
   // Epilogue: C[row][col] = alpha * acc + beta * C.
-  const bool beta0 = (p.beta == T(0));
 #pragma unroll
   for (int i = 0; i < IT; ++i) {
 #pragma unroll
@@ -224,7 +245,7 @@ __global__ __launch_bounds__(NT, 2) void gemm_kernel(GemmArgs<T> p) {
         const int64_t col = n0 + wn * (BN / 2) + j * 16 + lr;
         if (col < p.N) {
           T r = p.alpha * acc[i][j][v];
-          if (!beta0) r += p.beta * crow[col];
+          if (!beta0) r += p.beta * (PC ? cpre[PC ? i : 0][v][PC ? j : 0] : crow[col]);
           crow[col] = r;
         }
       }
