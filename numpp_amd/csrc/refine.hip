@@ -174,7 +174,9 @@ __global__ void count_sweep_kernel(int64_t count, const int32_t* active, int32_t
 // 16-byte loads); the first pass's A loads are issued before the X staging, so
 // that traffic is in flight while X arrives; its 32 partial sums per lane are
 // combined with an xor step and a halving butterfly and written as one
-// partial per (row, column, chunk). Pass 2 (residual_finish_kernel) adds the
+// partial per (row, column, chunk). The staged X keeps the two rows a lane
+// reads (16 doubles) together in a 144-byte slot, so the lanes' 16-byte LDS
+// reads start 36 banks apart and never collide. Pass 2 (residual_finish_kernel) adds the
 // partials in chunk order (deterministic), forms r = b - sum and the per-column
 // max |r|. The block of matrix rows with index 0 also reports max |x| per column.
 constexpr int RES_RPW = 4, RES_ROWS = 4 * RES_RPW, RES_JC = 512, RES_NC = 8;
@@ -191,7 +193,8 @@ __global__ __launch_bounds__(256) void residual_partial_kernel(int n, int nrhs, 
   const double* __restrict__ A = Ab + mat * (int64_t)n * n;
   const double* __restrict__ X = Xb + mat * (int64_t)n * nrhs;
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-  __shared__ double xs[RES_JC][RES_NC];
+  constexpr int XP = 2 * RES_NC + 2;  // doubles per row-pair slot
+  __shared__ double xs[RES_JC / 2][XP];
   const bool even = (n & 1) == 0;
   double a0[RES_JC / 128][RES_RPW], a1[RES_JC / 128][RES_RPW];
   auto load_a = [&](int row0) {
@@ -225,7 +228,7 @@ __global__ __launch_bounds__(256) void residual_partial_kernel(int n, int nrhs, 
     const int e = t + 256 * q;
     const int j = e / RES_NC, c = e % RES_NC;
     const double v = (jc + j < n && c < nc) ? X[(int64_t)(jc + j) * nrhs + c0 + c] : 0.0;
-    xs[j][c] = v;
+    xs[j >> 1][(j & 1) * RES_NC + c] = v;
     xm = fmax(xm, fabs(v));
   }
   if (blockIdx.x == 0) {
@@ -246,10 +249,10 @@ __global__ __launch_bounds__(256) void residual_partial_kernel(int n, int nrhs, 
       for (int c = 0; c < RES_NC; ++c) acc[r][c] = 0.0;
 #pragma unroll
     for (int sub = 0; sub < RES_JC / 128; ++sub) {
-      const int jl = sub * 128 + 2 * lane;
+      const int jp = sub * 64 + lane;
 #pragma unroll
       for (int c = 0; c < RES_NC; ++c) {
-        const double x0 = xs[jl][c], x1 = xs[jl + 1][c];
+        const double x0 = xs[jp][c], x1 = xs[jp][RES_NC + c];
 #pragma unroll
         for (int r = 0; r < RES_RPW; ++r) acc[r][c] = fma(a1[sub][r], x1, fma(a0[sub][r], x0, acc[r][c]));
       }
