@@ -17,6 +17,29 @@ typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 
 constexpr int kWave = 64;
 
+// 16 x 16 x 4 MFMA in FP64 or FP32. Operands: one A element and one B element
+// per lane (A[i][k]: lane = i + 16k, B[k][j]: lane = j + 16k); result: four
+// C elements per lane, column lane % 16, rows row(lane / 16, v), v = 0..3
+// (lane/16 + 4v for the FP64 instruction, 4 (lane/16) + v for the FP32 one).
+template <class T>
+struct Mfma;
+template <>
+struct Mfma<double> {
+  typedef v4d acc_t;
+  static __device__ __forceinline__ int row(int lk, int v) { return lk + 4 * v; }
+  static __device__ __forceinline__ acc_t run(double a, double b, acc_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+};
+template <>
+struct Mfma<float> {
+  typedef v4f acc_t;
+  static __device__ __forceinline__ int row(int lk, int v) { return 4 * lk + v; }
+  static __device__ __forceinline__ acc_t run(float a, float b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+};
+
 // ---- error reporting (thread-local message, C-ABI status codes) ----
 void set_error(const char* fmt, ...);
 int hip_fail(hipError_t e, const char* what, const char* file, int line);

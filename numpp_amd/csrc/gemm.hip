@@ -1,9 +1,7 @@
 // Row-major batched GEMM on gfx950 MFMA: C = alpha * A * B + beta * C.
 //
-// FP64 uses v_mfma_f64_16x16x4f64, FP32 uses v_mfma_f32_16x16x4f32; both take
-// one A element and one B element per lane (A[i][k]: lane = i + 16k, B[k][j]:
-// lane = j + 16k) and return four C elements per lane, column lane%16, rows
-// lane/16 + 4v for the FP64 instruction and 4*(lane/16) + v for the FP32 one.
+// FP64 uses v_mfma_f64_16x16x4f64, FP32 uses v_mfma_f32_16x16x4f32 (Mfma<T>,
+// with its operand and result layouts, in common.hpp).
 //
 // Workgroup: 256 threads = 4 waves in a 2 x 2 grid; macro tile BM x BN = 128 x 128,
 // each wave owns a 64 x 64 sub-tile (4 x 4 MFMA tiles, 16 accumulators).
@@ -32,25 +30,6 @@ namespace {
 
 constexpr int BK = 16, NT = 256;
 constexpr int PAD = 16;
-
-template <class T>
-struct Mfma;
-template <>
-struct Mfma<double> {
-  typedef v4d acc_t;
-  static __device__ __forceinline__ int row(int lk, int v) { return lk + 4 * v; }
-  static __device__ __forceinline__ acc_t run(double a, double b, acc_t c) {
-    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-  }
-};
-template <>
-struct Mfma<float> {
-  typedef v4f acc_t;
-  static __device__ __forceinline__ int row(int lk, int v) { return 4 * lk + v; }
-  static __device__ __forceinline__ acc_t run(float a, float b, acc_t c) {
-    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
-  }
-};
 
 __device__ __forceinline__ void tile_coords(int bid, int tiles_m, int tiles_n, int& tm, int& tn) {
   const int total = tiles_m * tiles_n;

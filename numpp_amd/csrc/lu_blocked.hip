@@ -34,61 +34,142 @@ namespace npph {
 
 namespace {
 
+// One 16 x 16 MFMA tile from LDS: X[0..16)[0..K) Y[0..K)[0..16) for X and Y
+// with row stride LD (K a multiple of 4).
+template <class T, int LD, int K>
+__device__ __forceinline__ typename Mfma<T>::acc_t lds_tile(const T* x, const T* y, int lane) {
+  typename Mfma<T>::acc_t acc = {0, 0, 0, 0};
+  const int r = lane & 15, kk = lane >> 4;
+#pragma unroll
+  for (int k0 = 0; k0 < K; k0 += 4) acc = Mfma<T>::run(x[r * LD + k0 + kk], y[(k0 + kk) * LD + r], acc);
+  return acc;
+}
+
+template <class T, int LD>
+__device__ __forceinline__ void lds_store_tile(const typename Mfma<T>::acc_t& acc, T* dst, T scale, int lane) {
+#pragma unroll
+  for (int v = 0; v < 4; ++v) dst[Mfma<T>::row(lane >> 4, v) * LD + (lane & 15)] = scale * acc[v];
+}
+
 // B <- L^-1 B for the unit-lower s x s block L at positions (r0, r0), s <= 64,
-// and the columns [c0, c0+nc) (rows through perm). One thread per column (TT threads share one copy of L,
-// stored transposed in LDS), column-oriented substitution: step p subtracts
-// L[q][p] x[p] from every later row q, so the dependent chain is one FMA per
-// step rather than a q-term dot product per row. TT = 64, 128 or 256 threads,
-// the narrowest that covers nc (idle threads would hold registers and LDS).
-template <class T, int TT>
-__global__ __launch_bounds__(TT) void trsm_leaf_kernel(T* Abase, int64_t strideA, int n, int r0, int s, int c0,
+// and the columns [c0, c0+nc) (rows through perm). Each workgroup (256
+// threads, 64 columns) forms M = L^-1 in LDS and applies it on the matrix core,
+// so no step of a 64-long substitution chain is left on the critical path:
+//   - the four 16 x 16 diagonal blocks of M by substitution (one thread per
+//     column of a block, 15 steps);
+//   - block doubling, M21 = -M22 (L21 M11), for the 16-row pairs and then the
+//     32-row pair, on 16 x 16 MFMA tiles;
+//   - the tile's 64 x 64 columns of B become M B (MFMA, M lower triangular).
+// L is padded to 64 with the identity, so M's padding rows and columns are the
+// identity's and the padding rows of B (read as zero) are never written. Each
+// level's T = L21 M11 overwrites the L21 block it was formed from, and the B
+// tile takes L's place once M is complete (two 64 x 64 arrays of LDS).
+// Applying the explicit inverse instead of substituting changes only the
+// rounding of B, within the usual bound for a unit-lower L whose entries are at
+// most 1 in magnitude (partial pivoting).
+template <class T>
+__global__ __launch_bounds__(256) void trsm_inv_kernel(T* Abase, int64_t strideA, int n, int r0, int s, int c0,
                                                        int nc, const int32_t* permb, const int32_t* mask) {
   const int64_t mat = blockIdx.y;
   if (mask && mask[mat] == 0) return;
   T* A = Abase + mat * strideA;
-  __shared__ T LT[LEAF][LEAF + 1];  // LT[p][q] = L[q][p]
-  __shared__ int rowp[LEAF];        // physical rows of positions r0 .. r0 + s - 1
+  constexpr int LD = LEAF + 1;
+  __shared__ T Ls[LEAF * LD];  // L padded with the identity; level products; the B tile
+  __shared__ T Ms[LEAF * LD];  // M = L^-1
+  __shared__ int rowp[LEAF];   // physical rows of positions r0 .. r0 + s - 1
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int32_t* perm = permb + mat * n;
-  for (int i = threadIdx.x; i < LEAF; i += TT) rowp[i] = perm[r0 + min(i, s - 1)];
+  if (t < LEAF) rowp[t] = perm[r0 + min(t, s - 1)];
+  __syncthreads();
+  // this workgroup's 64 columns of B (registers until M is ready) and L
+  const int cb = blockIdx.x * 64;
+  constexpr int PER = LEAF * 64 / 256;
+  T bv[PER], lv[PER];
+#pragma unroll
+  for (int u = 0; u < PER; ++u) {
+    const int e = t + 256 * u, i = e >> 6, j = e & 63;
+    const T* row = A + (int64_t)rowp[i] * n;
+    bv[u] = (i < s && cb + j < nc) ? row[c0 + cb + j] : T(0);
+    lv[u] = row[r0 + min(j, s - 1)];  // clamped: a real entry of a real row
+  }
+#pragma unroll
+  for (int u = 0; u < PER; ++u) {
+    const int e = t + 256 * u, i = e >> 6, j = e & 63;
+    Ls[i * LD + j] = (j < i && i < s) ? lv[u] : (i == j ? T(1) : T(0));
+    Ms[i * LD + j] = T(0);  // M's blocks above the diagonal stay zero
+  }
+  __syncthreads();
+  // diagonal 16 x 16 blocks of M: thread (b, j) substitutes unit vector e_j
+  if (t < 64) {
+    const int b = t >> 4, j = t & 15;
+    const T* lb = Ls + 16 * b * LD + 16 * b;
+    T x[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) x[q] = q == j ? T(1) : T(0);
+#pragma unroll
+    for (int p = 0; p < 15; ++p) {
+      const T xp = x[p];
+#pragma unroll
+      for (int q = p + 1; q < 16; ++q) x[q] = x[q] - lb[q * LD + p] * xp;
+    }
+    T* mb = Ms + 16 * b * LD + 16 * b;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) mb[q * LD + j] = x[q];
+  }
+  __syncthreads();
+  // 16-row pairs (rows 16..31 x columns 0..15, rows 48..63 x columns 32..47):
+  // waves 0 and 1 form T = L21 M11 (over L21, which only this wave reads),
+  // then M21 = -M22 T
+  if (wave < 2) {
+    const int o = 32 * wave;
+    auto acc = lds_tile<T, LD, 16>(Ls + (o + 16) * LD + o, Ms + o * LD + o, lane);
+    lds_store_tile<T, LD>(acc, Ls + (o + 16) * LD + o, T(1), lane);
+  }
+  __syncthreads();
+  if (wave < 2) {
+    const int o = 32 * wave;
+    auto acc = lds_tile<T, LD, 16>(Ms + (o + 16) * LD + o + 16, Ls + (o + 16) * LD + o, lane);
+    lds_store_tile<T, LD>(acc, Ms + (o + 16) * LD + o, T(-1), lane);
+  }
+  __syncthreads();
+  // the 32-row pair (rows 32..63 x columns 0..31): wave w owns 16 x 16 tile
+  // (w / 2, w % 2) of T = L21 M11 (stored over L21 once every wave has read
+  // it) and then of M21 = -M22 T
+  const int ti = wave >> 1, tj = wave & 1;
+  {
+    auto acc = lds_tile<T, LD, 32>(Ls + (32 + 16 * ti) * LD, Ms + 16 * tj, lane);
+    __syncthreads();
+    lds_store_tile<T, LD>(acc, Ls + (32 + 16 * ti) * LD + 16 * tj, T(1), lane);
+  }
   __syncthreads();
   {
-    // coalesced clamped loads of L (row i, column j), transposed into LDS
-    constexpr int PER = LEAF * LEAF / TT;
-    T v[PER];
+    // row block ti of the lower-triangular M22 has nonzeros in columns < 16 (ti + 1)
+    typename Mfma<T>::acc_t acc;
+    if (ti == 0) acc = lds_tile<T, LD, 16>(Ms + 32 * LD + 32, Ls + 32 * LD + 16 * tj, lane);
+    else acc = lds_tile<T, LD, 32>(Ms + 48 * LD + 32, Ls + 32 * LD + 16 * tj, lane);
+    lds_store_tile<T, LD>(acc, Ms + (32 + 16 * ti) * LD + 16 * tj, T(-1), lane);
+  }
+  // the B tile over L once every wave has read its T tile
+  __syncthreads();
 #pragma unroll
-    for (int u = 0; u < PER; ++u) {
-      const int e = threadIdx.x + u * TT, i = e / LEAF, j = e - i * LEAF;
-      v[u] = A[(int64_t)rowp[i] * n + r0 + min(j, s - 1)];
-    }
-#pragma unroll
-    for (int u = 0; u < PER; ++u) {
-      const int e = threadIdx.x + u * TT, i = e / LEAF, j = e - i * LEAF;
-      LT[j][i] = (i < s && j < s) ? v[u] : T(0);
-    }
+  for (int u = 0; u < PER; ++u) {
+    const int e = t + 256 * u, i = e >> 6, j = e & 63;
+    Ls[i * LD + j] = bv[u];
   }
   __syncthreads();
-  const int c = blockIdx.x * TT + threadIdx.x;
-  if (c >= nc) return;
-  T* base = A + c0 + c;  // position r0 + q of this column at base + rowp[q] * n
-  T x[LEAF];
+  // M B: wave w owns columns [16w, 16w + 16) of the tile, all four row blocks
+  const int col = cb + 16 * wave + (lane & 15);
+  auto put = [&](const typename Mfma<T>::acc_t& acc, int rb) {
 #pragma unroll
-  for (int q = 0; q < LEAF; ++q) x[q] = base[(int64_t)rowp[q] * n];
-#pragma unroll
-  for (int q = 0; q < LEAF; ++q) x[q] = q < s ? x[q] : T(0);
-#pragma unroll
-  for (int p = 0; p < LEAF - 1; ++p) {
-    // An opaque zero that depends on a recent result: step p's L reads cannot
-    // be hoisted far above it (hoisting all 2016 LDS reads spills the kernel).
-    int dep = 0;
-    if (p >= 1) asm volatile("v_mov_b32 %0, 0" : "=v"(dep) : "v"(x[sizeof(T) == 4 ? p - 1 : p]));
-    const T* lp = &LT[p][0] + dep;
-    const T xp = x[p];
-#pragma unroll
-    for (int q = p + 1; q < LEAF; ++q) x[q] = x[q] - lp[q] * xp;
-  }
-#pragma unroll
-  for (int q = 0; q < LEAF; ++q)
-    if (q < s) base[(int64_t)rowp[q] * n] = x[q];
+    for (int v = 0; v < 4; ++v) {
+      const int i = 16 * rb + Mfma<T>::row(lane >> 4, v);
+      if (i < s && col < nc) A[(int64_t)rowp[i] * n + c0 + col] = acc[v];
+    }
+  };
+  put(lds_tile<T, LD, 16>(Ms, Ls + 16 * wave, lane), 0);
+  put(lds_tile<T, LD, 32>(Ms + 16 * LD, Ls + 16 * wave, lane), 1);
+  put(lds_tile<T, LD, 48>(Ms + 32 * LD, Ls + 16 * wave, lane), 2);
+  put(lds_tile<T, LD, 64>(Ms + 48 * LD, Ls + 16 * wave, lane), 3);
 }
 
 // Packed row order from a factorisation whose rows never moved: row p of
@@ -202,15 +283,8 @@ struct Rec {
     if (nc <= 0) return NPPH_OK;
     if (sz <= LEAF) {
       ProfScope ps(kProfMoveTrsm, (double)count * sz * sz * nc, 2.0 * sizeof(T) * count * (double)sz * nc, s);
-      if (nc <= 64)
-        hipLaunchKernelGGL((trsm_leaf_kernel<T, 64>), dim3(1u, (unsigned)count), dim3(64), 0, s, A, strideA, n, r0,
-                           sz, c0, nc, perm, mask);
-      else if (nc <= 128)
-        hipLaunchKernelGGL((trsm_leaf_kernel<T, 128>), dim3(1u, (unsigned)count), dim3(128), 0, s, A, strideA, n,
-                           r0, sz, c0, nc, perm, mask);
-      else
-        hipLaunchKernelGGL((trsm_leaf_kernel<T, 256>), dim3((unsigned)ceil_div(nc, 256), (unsigned)count),
-                           dim3(256), 0, s, A, strideA, n, r0, sz, c0, nc, perm, mask);
+      hipLaunchKernelGGL((trsm_inv_kernel<T>), dim3((unsigned)ceil_div(nc, 64), (unsigned)count), dim3(256), 0, s, A,
+                         strideA, n, r0, sz, c0, nc, perm, mask);
       NPPH_HIP(hipGetLastError());
       return NPPH_OK;
     }
