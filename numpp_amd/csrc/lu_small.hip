@@ -16,7 +16,8 @@
 //     (|a_k|, pos) with NumPP's rule (largest magnitude, then smallest current
 //     position, linalg.cpp:55-56; a NaN at position k is kept, a NaN below it is
 //     never chosen), divides the column by the pivot (linalg.cpp:65-66) and
-//     publishes the multipliers, the pivot position and the pivot lane in LDS;
+//     publishes the multipliers, the pivot position and the pivot lane in LDS
+//     (lane -1 for an exact-zero pivot: the factorisation stops there);
 //   - one workgroup barrier (the LDS slots alternate between even and odd
 //     steps, so the next owner never overwrites what a slower wave still reads);
 //   - every wave with columns right of k updates them (linalg.cpp:67), reading
@@ -54,6 +55,7 @@ struct SmallShared {
   double L[N][N + 1];    // strictly lower factor by logical row (zeros elsewhere)
   double U[N][N + 1];    // strictly upper factor by logical row, column k / u_kk
   double D[N];           // diagonal of U
+  double Dr[N];          // its reciprocals
   int perm[N];           // original row at each logical position
 };
 
@@ -63,7 +65,7 @@ __device__ __forceinline__ void factor(SmallShared<N>& sh, double (&a)[Small<N>:
   constexpr int CPW = Small<N>::CPW;
   bool stop = false;
   auto step = [&]<int K>() __attribute__((always_inline)) {
-    if (stop || K >= n) return;  // uniform over the workgroup: n and sing are
+    if (stop || K >= n) return;  // uniform over the workgroup (every wave stops at the same step)
     constexpr int O = K / CPW, CO = K % CPW, BUF = K & 1;
     double l = 0.0;
     int wl = 0, p = K;
@@ -84,22 +86,26 @@ __device__ __forceinline__ void factor(SmallShared<N>& sh, double (&a)[Small<N>:
       sh.l[BUF][lane] = l;
       if (lane == 0) {
         sh.p[BUF] = p;
-        sh.wl[BUF] = wl;
+        sh.wl[BUF] = v == 0.0 ? -1 : wl;
         sh.piv[K] = p;
         if (v == 0.0) sh.sing = K;
       }
-    }
-    __syncthreads();
-    if constexpr (W != O) {
+      __syncthreads();
+      if (v == 0.0) {
+        stop = true;
+        return;
+      }
+    } else {
+      __syncthreads();
       p = sh.p[BUF];
       wl = sh.wl[BUF];
+      if (wl < 0) {
+        stop = true;
+        return;
+      }
       if (pos == p) pos = K;
       else if (pos == K) pos = p;
       l = sh.l[BUF][lane];
-    }
-    if (sh.sing >= 0) {
-      stop = true;
-      return;
     }
     // columns of this wave right of the pivot column
 #pragma unroll
@@ -142,22 +148,35 @@ __device__ __forceinline__ void wave_body(const UniformBatch& u, SmallShared<N>&
     if (lane < ks) u.ipiv[mat * n + lane] = sh.piv[lane];
   }
   if (u.LU && lane < n && pos < n) {
-    double* o = u.LU + mat * u.strideA + (int64_t)pos * n;
+    double* o = u.LU + mat * u.strideA + (int64_t)pos * n + W * CPW;
+    if (n == N && ((reinterpret_cast<uintptr_t>(u.LU) | (uintptr_t)u.strideA * 8) & 15) == 0) {
+      // full rows: this lane's CPW columns as 16-byte vectors
 #pragma unroll
-    for (int c = 0; c < CPW; ++c)
-      if (W * CPW + c < n) o[W * CPW + c] = a[c];
+      for (int c = 0; c < CPW; c += 2) *reinterpret_cast<double2*>(o + c) = make_double2(a[c], a[c + 1]);
+    } else {
+#pragma unroll
+      for (int c = 0; c < CPW; ++c)
+        if (W * CPW + c < n) o[c] = a[c];
+    }
   }
   if (singular || u.nrhs <= 0 || !u.X) return;  // uniform over the workgroup
   // Solve operands in LDS by logical row, shaped so the per-step reads need no
   // lane masks: L strictly lower (zeros elsewhere) and V = U D^-1 strictly
-  // upper (column k of U divided by u_kk). Back substitution then runs on
-  // z = D x: z_i -= V[i][k] z_k, and x_i = z_i / u_ii at the end. (linalg.cpp:
-  // 83-88 divides x_k by u_kk before it is used; dividing U's column instead
-  // gives the same solution within rounding.)
+  // upper (column k of U times 1 / u_kk, one division per column). Back
+  // substitution then runs on z = D x: z_i -= V[i][k] z_k, and x_i = z_i / u_ii
+  // at the end. (linalg.cpp:83-88 divides x_k by u_kk before it is used;
+  // scaling U's column instead gives the same solution within rounding.)
   if (lane < N) {
+    // the lane holding its row's diagonal entry (at most one column here)
+    double dv = 1.0;
+    bool has = false;
 #pragma unroll
     for (int c = 0; c < CPW; ++c)
-      if (W * CPW + c == pos) sh.D[pos] = a[c];
+      if (W * CPW + c == pos) { dv = a[c]; has = true; }
+    if (has) {
+      sh.D[pos] = dv;
+      sh.Dr[pos] = 1.0 / dv;
+    }
     if (W == 0) sh.perm[pos] = lane;
   }
   __syncthreads();
@@ -166,7 +185,7 @@ __device__ __forceinline__ void wave_body(const UniformBatch& u, SmallShared<N>&
     for (int c = 0; c < CPW; ++c) {
       const int j = W * CPW + c;
       sh.L[pos][j] = j < pos ? a[c] : 0.0;
-      sh.U[pos][j] = j > pos ? a[c] / sh.D[j] : 0.0;
+      sh.U[pos][j] = j > pos ? a[c] * sh.Dr[j] : 0.0;
     }
   }
   __syncthreads();
@@ -241,7 +260,7 @@ int small_lu_launch(const UniformBatch& u, hipStream_t st) {
                8.0 * u.count * (n * n + 2 * n * u.nrhs + (u.LU ? n * n : 0)), st);
   // a single right-hand side (or none) solves one register column per pass
   const bool one = u.nrhs <= 1;
-auto launch = [&]<int NV>() {
+  auto launch = [&]<int NV>() {
     if (one) hipLaunchKernelGGL((lu_small_kernel<NV, 1>), grid, dim3(Small<NV>::NT), 0, st, u);
     else hipLaunchKernelGGL((lu_small_kernel<NV, 8>), grid, dim3(Small<NV>::NT), 0, st, u);
   };
