@@ -256,6 +256,22 @@ __device__ __forceinline__ void lanes8_argmax(V& v, int& pos) {
 // reduced as well.
 template <class V>
 __device__ __forceinline__ void wave_argmax(V& v, int& pos) {
+  if constexpr (sizeof(V) == 8) {
+    // FP64 values are ranked by their FP32 roundings first (half the DPP
+    // traffic and compares): rounding to nearest never reverses an order, so a
+    // single lane holding the FP32 maximum holds the FP64 maximum too. Only
+    // when several lanes share the FP32 maximum (equal or nearly equal values)
+    // is the exact FP64 argmax below run.
+    const float f = (float)v;
+    const float fm = wave_max(f);
+    const uint64_t mf = __ballot(f == fm);
+    if (__popcll(mf) == 1) {
+      const int wl = (int)__builtin_ctzll(mf);
+      v = readlane<V>(v, wl);
+      pos = __builtin_amdgcn_readlane(pos, wl);
+      return;
+    }
+  }
   const V wm = wave_max(v);
   const uint64_t m = __ballot(v == wm);
   int wp;
