@@ -7,7 +7,8 @@
 // right-hand-side columns: it accumulates the products of its factor strip
 // with the blocks solved before it (L_bj Y_j for j < b forward, U_bj X_j for
 // j > b backward) on MFMA 16x16x4 as their results appear, solves its 64 x 64
-// diagonal block, writes its rows and publishes a per-block ready flag.
+// diagonal block and writes its rows, to X and, tagged, to a staging area that
+// the later blocks poll.
 //
 // Workgroups take their (step, system, chunk) from a ticket counter in step
 // order, so a workgroup only ever waits for blocks whose workgroups took
@@ -15,12 +16,16 @@
 // ticket never waits, and the sweep always drains. Waits are bounded (a
 // workgroup that gives up records it in an error word and goes on).
 //
-// Visibility across XCDs (each has its own L2) without cache-wide fences: the
-// published rows are written and read with device-scope (agent) relaxed
-// atomic stores and loads, which the hardware keeps coherent across the L2s
-// line by line; a writer waits until its row stores are acknowledged and the
-// workgroup barrier has passed before it stores the flag, and readers issue
-// their row loads only after the flag load has returned the ready value. An
+// Publication without flags or fences: every solved entry goes to the staging
+// area as 64-bit words that carry the value (FP32: one word; FP64: two words,
+// one per half) in their low 32 bits and the sweep's tag (1 forward, 2
+// backward) in their high 32 bits, written and read with device-scope (agent)
+// relaxed atomic stores and loads, which the hardware keeps coherent across
+// the XCDs' L2s word by word. A reader polls the words it needs until every
+// tag is the sweep's, so the poll is the data read: one trip through the
+// memory side per dependency instead of flag-then-data, and the writer never
+// waits for its stores to be acknowledged. The staging area is cleared before
+// each forward sweep, so no word carries a tag it was not written with. An
 // acquire/release pair would instead write back or invalidate the whole L2 of
 // the XCD on every poll and publish.
 #include <type_traits>
@@ -36,8 +41,11 @@ constexpr int MB = 64;   // rows per block
 constexpr int MT = 256;  // threads per workgroup (4 waves: 16 block rows each)
 constexpr uint32_t kSpinLimit = 1u << 22;
 // sync words: [0] forward tickets, [1] backward tickets, [2] error, [3] unused,
-// then one ready flag per (system, chunk, block)
+// then the staging area: per (system, chunk, block), MB x CW entries of
+// WPV(T) tagged 64-bit words
 constexpr int kSyncHead = 4;
+template <class T>
+constexpr int WPV = sizeof(T) / 4;  // tagged words per value
 
 struct MwRef {
   const double* R = nullptr;      // residual (forward right-hand side)
@@ -64,10 +72,31 @@ __device__ __forceinline__ T coherent_load(const T* p) {
   return __builtin_bit_cast(T, __hip_atomic_load(reinterpret_cast<const bits_t<T>*>(p), __ATOMIC_RELAXED,
                                                  __HIP_MEMORY_SCOPE_AGENT));
 }
+
+// A value's tagged words: (32 value bits) | tag << 32, one word per 32 bits.
 template <class T>
-__device__ __forceinline__ void coherent_store(T* p, T v) {
-  __hip_atomic_store(reinterpret_cast<bits_t<T>*>(p), __builtin_bit_cast(bits_t<T>, v), __ATOMIC_RELAXED,
-                     __HIP_MEMORY_SCOPE_AGENT);
+__device__ __forceinline__ void tagged_store(uint64_t* w, T v, uint32_t tag) {
+  const uint64_t hi = (uint64_t)tag << 32;
+  if constexpr (sizeof(T) == 4) {
+    __hip_atomic_store(w, hi | __builtin_bit_cast(uint32_t, v), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  } else {
+    const uint64_t b = __builtin_bit_cast(uint64_t, v);
+    __hip_atomic_store(w, hi | (b & 0xffffffffu), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(w + 1, hi | (b >> 32), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+// Loads a value's tagged words; false while any of them lacks the tag.
+template <class T>
+__device__ __forceinline__ bool tagged_load(const uint64_t* w, uint32_t tag, T& v) {
+  const uint64_t a = __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if constexpr (sizeof(T) == 4) {
+    v = __builtin_bit_cast(float, (uint32_t)a);
+    return (uint32_t)(a >> 32) == tag;
+  } else {
+    const uint64_t b = __hip_atomic_load(w + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    v = __builtin_bit_cast(double, (a & 0xffffffffu) | (b << 32));
+    return (uint32_t)(a >> 32) == tag && (uint32_t)(b >> 32) == tag;
+  }
 }
 
 __global__ void mw_clear_kernel(uint32_t* sync, int64_t words) {
@@ -113,8 +142,9 @@ __global__ __launch_bounds__(MT) void sweep_mw_kernel(int64_t count, int n, int 
   // factor row of position p (factors whose rows never moved carry a row map)
   const int32_t* rowmap = rowmapb ? rowmapb + mat * n : nullptr;
   auto rowp = [&](int p) -> int64_t { return rowmap ? rowmap[p] : p; };
-  uint32_t* flags = sync + kSyncHead + (mat * nz + z) * nblk;
-  const uint32_t ready = LOWER ? 1u : 2u;
+  // this (system, chunk)'s staging blocks
+  uint64_t* stg = reinterpret_cast<uint64_t*>(sync + kSyncHead) + (mat * nz + z) * nblk * (MB * CW * WPV<T>);
+  const uint32_t tag = LOWER ? 1u : 2u;
 
   // ---- diagonal block into LDS (clamped, coalesced; entries past bs zeroed)
   {
@@ -189,28 +219,33 @@ __global__ __launch_bounds__(MT) void sweep_mw_kernel(int64_t count, int n, int 
         }
     }
   };
-  auto wait_ready = [&](int j) {
-    if (t == 0) {
-      uint32_t spins = 0;
-      while (__hip_atomic_load(&flags[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != ready) {
-        __builtin_amdgcn_s_sleep(1);
-        if (++spins > kSpinLimit) {
-          __hip_atomic_store(&sync[2], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          break;
-        }
-      }
-    }
-    __syncthreads();
-  };
   acc_t acc = {};
+  // block j's solved entries from the staging area: every thread polls its
+  // entries until the whole workgroup holds tagged ones (bounded)
+  constexpr int EPT = MB * CW / MT;  // entries per thread
   auto consume = [&](const T (&a)[4][4], int j, int buf) {
-    wait_ready(j);
-    {
-      const int q0 = j * MB, qs = min(MB, n - q0);
-      for (int e = t; e < MB * 16; e += MT) {
-        const int rr = e >> 4, cc = e & 15;
-        xs[buf][rr][cc] = (rr < qs && cc < nc) ? coherent_load(&X[(int64_t)(q0 + rr) * nrhs + c0 + cc]) : T(0);
+    const uint64_t* sj = stg + (int64_t)j * (MB * CW * WPV<T>);
+    T val[EPT];
+    uint32_t spins = 0;
+    for (;;) {
+      bool ok = true;
+#pragma unroll
+      for (int u = 0; u < EPT; ++u) ok = tagged_load<T>(sj + (t + MT * u) * WPV<T>, tag, val[u]) && ok;
+      if (__syncthreads_and(ok)) break;
+      if (++spins > kSpinLimit) {
+        if (t == 0) __hip_atomic_store(&sync[2], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        break;
       }
+      __builtin_amdgcn_s_sleep(1);
+    }
+#pragma unroll
+    for (int u = 0; u < EPT; ++u) {
+      const int e = t + MT * u, rr = e / CW, cc = e % CW;
+      xs[buf][rr][cc] = val[u];
+    }
+    if constexpr (CW < 16) {
+      // columns CW..15 of the MFMA operand stay zero
+      for (int e = t; e < MB * (16 - CW); e += MT) xs[buf][e / (16 - CW)][CW + e % (16 - CW)] = T(0);
     }
     __syncthreads();
 #pragma unroll
@@ -279,14 +314,23 @@ __global__ __launch_bounds__(MT) void sweep_mw_kernel(int64_t count, int n, int 
     }
   }
 
-  // ---- rows out, then publish
+  // ---- rows out: published (tagged) first, for the blocks that wait on them
+  if (st + 1 < nblk) {
+    uint64_t* sb = stg + (int64_t)b * (MB * CW * WPV<T>);
+#pragma unroll
+    for (int j = 0; j < CPW; ++j) {
+      const int c = wave + 4 * j;
+      // rows past bs and columns past nc publish zeros
+      tagged_store<T>(sb + (lane * CW + c) * WPV<T>, (lane < bs && c < nc) ? x[j] : T(0), tag);
+    }
+  }
   if (lane < bs) {
     const int i = r0 + lane;
 #pragma unroll
     for (int j = 0; j < CPW; ++j) {
       const int c = wave + 4 * j;
       if (c < nc) {
-        coherent_store(&X[(int64_t)i * nrhs + c0 + c], x[j]);
+        X[(int64_t)i * nrhs + c0 + c] = x[j];
         if constexpr (REF && !LOWER) {
           const double dd = (double)x[j] / ref.scale[mat * nrhs + c0 + c];
           double* xr = ref.X + mat * strideX + (int64_t)i * nrhs + c0 + c;
@@ -294,11 +338,6 @@ __global__ __launch_bounds__(MT) void sweep_mw_kernel(int64_t count, int n, int 
         }
       }
     }
-  }
-  if (st + 1 < nblk) {
-    __builtin_amdgcn_s_waitcnt(0);  // this thread's row stores are acknowledged
-    __syncthreads();
-    if (t == 0) __hip_atomic_store(&flags[b], ready, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
 }
 
@@ -329,7 +368,7 @@ int getrs_mw(int64_t count, int n, int nrhs, const T* LU, int64_t strideA, const
              const double* R, const double* scale, double* X64, int first, hipStream_t s) {
   const int cw = nrhs <= 4 ? 4 : (nrhs <= 8 ? 8 : 16);
   const int nz = (int)ceil_div(nrhs, cw);
-  const int64_t words = kSyncHead + count * nz * ceil_div(n, MB);
+  const int64_t words = kSyncHead + 2 * count * nz * ceil_div(n, MB) * (MB * cw * (int64_t)(sizeof(T) / 4));
   uint32_t* sync = nullptr;
   NPPH_TRY(workspace(kWsSync, sizeof(uint32_t) * (size_t)words, (void**)&sync));
   hipLaunchKernelGGL(mw_clear_kernel, dim3((unsigned)ceil_div(words, 256)), dim3(256), 0, s, sync, words);
