@@ -170,11 +170,29 @@ __global__ __launch_bounds__(NTH) void bf16p_gemm_kernel(int64_t M, int64_t N, i
                                                          int64_t Mp, const int* __restrict__ eA,
                                                          const __bf16* __restrict__ PB, int64_t Np,
                                                          const int* __restrict__ eB, float* __restrict__ Cb,
-                                                         int64_t ldc, int64_t sC, int tiles_n, RowMaps maps) {
+                                                         int64_t ldc, int64_t sC, int tiles_m, int tiles_n,
+                                                         RowMaps maps) {
   __shared__ __bf16 As[2][NP][TM][KPAD];
   __shared__ __bf16 Bs[2][NP][TN][KPAD];
-  const int64_t bz = blockIdx.y;
-  const int tm = blockIdx.x / tiles_n, tn = blockIdx.x - tm * tiles_n;
+  // Work order: workgroups are dispatched round-robin over the 8 XCDs (each
+  // with its own L2); the permutation below gives XCD x a contiguous eighth of
+  // the (matrix, tile) sequence, so whole matrices stay on one XCD, and each
+  // matrix's tiles go in groups of four tile rows, so the workgroups running
+  // side by side share their A and B panels' K slices in that L2.
+  const int64_t total = gridDim.x;
+  int64_t idx = blockIdx.x;
+  if ((total & 7) == 0) idx = (idx >> 3) + (idx & 7) * (total >> 3);
+  const int64_t tiles = (int64_t)tiles_m * tiles_n;
+  const int64_t bz = idx / tiles;
+  int tm, tn;
+  {
+    constexpr int G = 4;
+    const int r = (int)(idx - bz * tiles);
+    const int g = r / (G * tiles_n), first = g * G, gs = min(tiles_m - first, G);
+    const int q = r - g * G * tiles_n;
+    tm = first + q % gs;
+    tn = q / gs;
+  }
   const int64_t m0 = (int64_t)tm * TM, n0 = (int64_t)tn * TN;
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int wm = wave >> 1, wn = wave & 1;
@@ -293,8 +311,8 @@ int bf16_update_np(int64_t M, int64_t N, int64_t K, const float* A, int64_t lda,
   hipLaunchKernelGGL(split_cols_kernel<NP>, dim3((unsigned)(Np / SPAN), (unsigned)batch), dim3(NTH), 0, s, N,
                      (int)K, Kp, B, ldb, sB, PB, Np, eB, maps);
   const int tiles_n = (int)(Np / TN);
-  hipLaunchKernelGGL(bf16p_gemm_kernel<NP>, dim3((unsigned)((Mp / TM) * tiles_n), (unsigned)batch), dim3(NTH), 0,
-                     s, M, N, Kp, PA, Mp, eA, PB, Np, eB, C, ldc, sC, tiles_n, maps);
+  hipLaunchKernelGGL(bf16p_gemm_kernel<NP>, dim3((unsigned)((Mp / TM) * tiles_n * batch)), dim3(NTH), 0, s, M, N, Kp,
+                     PA, Mp, eA, PB, Np, eB, C, ldc, sC, (int)(Mp / TM), tiles_n, maps);
   NPPH_HIP(hipGetLastError());
   return NPPH_OK;
 }
@@ -307,7 +325,7 @@ int gemm_bf16split_update(int64_t M, int64_t N, int64_t K, const float* A, int64
   if (M <= 0 || N <= 0 || batch <= 0) return NPPH_OK;
   if (K <= 0 || K > (1 << 20)) { set_error("bf16 split update: bad K=%ld", (long)K); return NPPH_ERR_INVALID; }
   if (batch > 65535) { set_error("bf16 split update: batch too large"); return NPPH_ERR_INVALID; }
-  if (ceil_div(M, TM) * ceil_div(N, TN) > (1LL << 31) - 1) {
+  if (ceil_div(M, TM) * ceil_div(N, TN) * batch > (1LL << 31) - 1) {
     set_error("bf16 split update: too many tiles");
     return NPPH_ERR_INVALID;
   }
