@@ -38,7 +38,19 @@ __global__ __launch_bounds__(256) void demote_kernel(int64_t count, int n, const
   float* o = Alo + gw * n;
   double s = 0.0;
   if ((n & 1) == 0) {
-    for (int j = 2 * lane; j < n; j += 128) {
+    int j = 2 * lane;
+    // four 16-byte loads in flight per lane before any is used
+    for (; j + 384 < n; j += 512) {
+      double2 v[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) v[u] = *reinterpret_cast<const double2*>(a + j + 128 * u);
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        *reinterpret_cast<float2*>(o + j + 128 * u) = make_float2((float)v[u].x, (float)v[u].y);
+        s += fabs(v[u].x) + fabs(v[u].y);
+      }
+    }
+    for (; j < n; j += 128) {
       const double2 v = *reinterpret_cast<const double2*>(a + j);
       *reinterpret_cast<float2*>(o + j) = make_float2((float)v.x, (float)v.y);
       s += fabs(v.x) + fabs(v.y);
