@@ -56,12 +56,16 @@ struct SmallShared {
   double U[N][N + 1];    // strictly upper factor by logical row, column k / u_kk
   double D[N];           // diagonal of U
   double Dr[N];          // its reciprocals
+  double y[N];           // L^-1 P b by logical row (single right-hand side)
   int perm[N];           // original row at each logical position
 };
 
 // Factorisation, as seen by wave W (all arguments uniform across the wave).
-template <int N, int W>
-__device__ __forceinline__ void factor(SmallShared<N>& sh, double (&a)[Small<N>::CPW], int& pos, int n, int lane) {
+// AUG: the last wave also carries b as an extra column (ab, by physical row),
+// so the forward substitution happens inside the elimination.
+template <int N, int W, bool AUG>
+__device__ __forceinline__ void factor(SmallShared<N>& sh, double (&a)[Small<N>::CPW], int& pos, int n, int lane,
+                                       double& ab) {
   constexpr int CPW = Small<N>::CPW;
   bool stop = false;
   auto step = [&]<int K>() __attribute__((always_inline)) {
@@ -115,6 +119,7 @@ __device__ __forceinline__ void factor(SmallShared<N>& sh, double (&a)[Small<N>:
         a[c] = fma(-l, u, a[c]);
       }
     }
+    if constexpr (AUG && W == Small<N>::NW - 1) ab = fma(-l, readlane_d(ab, wl), ab);
   };
   [&]<int... K>(std::integer_sequence<int, K...>) __attribute__((always_inline)) {
     (step.template operator()<K>(), ...);
@@ -122,7 +127,8 @@ __device__ __forceinline__ void factor(SmallShared<N>& sh, double (&a)[Small<N>:
 }
 
 // NR: right-hand-side columns held in registers per pass (1 for a single
-// right-hand side, else 8).
+// right-hand side, which then rides along the elimination as an extra column,
+// else 8).
 template <int N, int W, int NR>
 __device__ __forceinline__ void wave_body(const UniformBatch& u, SmallShared<N>& sh, int64_t mat, int lane) {
   constexpr int CPW = Small<N>::CPW, NW = Small<N>::NW;
@@ -137,8 +143,11 @@ __device__ __forceinline__ void wave_body(const UniformBatch& u, SmallShared<N>&
       a[c] = (lane < n && j < n) ? v : (lane == j ? 1.0 : 0.0);
     }
   }
+  constexpr bool AUG = NR == 1;
+  double ab = 0.0;
+  if (AUG && W == NW - 1 && u.nrhs > 0 && u.X && lane < n) ab = u.B[mat * u.strideB + lane];
   int pos = lane < N ? lane : (1 << 20);
-  factor<N, W>(sh, a, pos, n, lane);
+  factor<N, W, AUG>(sh, a, pos, n, lane, ab);
   const bool singular = sh.sing >= 0;
   if (W == 0 && lane == 0 && u.info) u.info[mat] = singular ? sh.sing + 1 : 0;
   if (W == 0 && u.ipiv) {
@@ -178,13 +187,14 @@ __device__ __forceinline__ void wave_body(const UniformBatch& u, SmallShared<N>&
       sh.Dr[pos] = 1.0 / dv;
     }
     if (W == 0) sh.perm[pos] = lane;
+    if (AUG && W == NW - 1) sh.y[pos] = ab;
   }
   __syncthreads();
   if (lane < N) {
 #pragma unroll
     for (int c = 0; c < CPW; ++c) {
       const int j = W * CPW + c;
-      sh.L[pos][j] = j < pos ? a[c] : 0.0;
+      if constexpr (!AUG) sh.L[pos][j] = j < pos ? a[c] : 0.0;
       sh.U[pos][j] = j > pos ? a[c] * sh.Dr[j] : 0.0;
     }
   }
@@ -203,15 +213,19 @@ __device__ __forceinline__ void wave_body(const UniformBatch& u, SmallShared<N>&
     asm volatile("s_mov_b32 %0, 0" : "=s"(z));
     const int nn = n + z;
     double y[NR];
+    if constexpr (AUG) {
+      y[0] = lane < n ? sh.y[lane] : 0.0;  // already L^-1 P b
+    } else {
 #pragma unroll
-    for (int q = 0; q < NR; ++q) y[q] = (lane < n && c0 + q < nrhs) ? B[(int64_t)src * nrhs + c0 + q] : 0.0;
-    // forward: y_i -= L[i][k] y_k (row order = logical position)
+      for (int q = 0; q < NR; ++q) y[q] = (lane < n && c0 + q < nrhs) ? B[(int64_t)src * nrhs + c0 + q] : 0.0;
+      // forward: y_i -= L[i][k] y_k (row order = logical position)
 #pragma unroll
-    for (int k = 0; k < N - 1; ++k) {
-      if (k + 1 >= nn) break;
-      const double f = sh.L[row][k];
+      for (int k = 0; k < N - 1; ++k) {
+        if (k + 1 >= nn) break;
+        const double f = sh.L[row][k];
 #pragma unroll
-      for (int q = 0; q < NR; ++q) y[q] = fma(-f, readlane_d(y[q], k), y[q]);
+        for (int q = 0; q < NR; ++q) y[q] = fma(-f, readlane_d(y[q], k), y[q]);
+      }
     }
     // back: z_i -= V[i][k] z_k for i < k
 #pragma unroll
