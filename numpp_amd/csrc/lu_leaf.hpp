@@ -68,8 +68,26 @@ __global__ __launch_bounds__(NTH) void leaf_kernel(LeafArgs<T> a) {
   // a global store inside the column loop would make every column barrier
   // wait for it (the barrier's release fence)
   __shared__ int piv_s[LEAF];
+  // Staging of the block's rows for panels whose inner-block rows are 128
+  // bytes (FP32 IB = 32, FP64 IB = 16) with 512 threads: the gather and the
+  // write-back move each row as eight 16-byte pieces read by eight
+  // neighbouring lanes (64 rows per wave instruction, whole cache lines),
+  // passing through LDS in 512-row chunks to and from the one-row-per-lane
+  // register layout. With one row per lane each wave instruction would touch
+  // 64 cache lines for 16 bytes apiece, which one CU's memory pipe serves at a
+  // third of the rate. Rows are padded to 144 bytes so the lanes' 16-byte LDS
+  // reads start 36 banks apart.
+  constexpr bool STG = NTH == 512 && IB * (int)sizeof(T) == 128;
+  constexpr int EPV = 16 / (int)sizeof(T);  // elements per 16-byte piece
+  constexpr int SW = IB + EPV;              // staging row stride (elements)
+  typedef T vec_t __attribute__((ext_vector_type(16 / sizeof(T))));
+  __shared__ __attribute__((aligned(16))) T stage[STG ? NTH : 1][STG ? SW : 1];
+  __shared__ int stage_phys[STG ? NTH : 1];
   extern __shared__ int rowof[];  // m entries: physical row of each leaf position
 
+  // 16-byte pieces need 16-byte aligned rows
+  const bool staged = ((reinterpret_cast<uintptr_t>(a.A) | (uintptr_t)(a.strideA * sizeof(T)) |
+                        (uintptr_t)(a.n * sizeof(T))) & 15) == 0;
   for (int i = t; i < m; i += NTH) rowof[i] = perm[k0 + i];
   // first exact-zero pivot step + 1 (0: none yet); every thread tracks it, as
   // every thread sees each step's pivot value
@@ -82,16 +100,48 @@ __global__ __launch_bounds__(NTH) void leaf_kernel(LeafArgs<T> a) {
     T v[R][IB];
     int pos[R], phys[R];  // current positions and (fixed) physical rows
     // ---- gather the block's columns from the rows' physical places
+    // (the row index is clamped, so the loads need no row guard; padding
+    // slots (pos >= m) are excluded from the pivot search and every write)
 #pragma unroll
     for (int s = 0; s < R; ++s) {
       const int p0 = jb + t + NTH * s;
       pos[s] = p0 < m ? p0 : (1 << 28);
       phys[s] = rowof[p0 < m ? p0 : 0];
-      const T* src = A + (int64_t)phys[s] * n + cb;
+    }
+    if (STG && FULL && staged) {
+      // piece t % 8 of rows t / 8 + 64 q of chunk s (the next chunk's loads
+      // are in flight while the current one passes through LDS)
+      vec_t piece[8];
+      auto load_chunk = [&](int s) __attribute__((always_inline)) {
 #pragma unroll
-      // the row index is clamped, so the loads need no row guard; padding
-      // slots (pos >= m) are excluded from the pivot search and every write
-      for (int c = 0; c < IB; ++c) v[s][c] = (FULL || c < ib) ? src[c] : T(0);
+        for (int q = 0; q < 8; ++q) {
+          const int pp = jb + NTH * s + (t >> 3) + 64 * q;
+          const int ph = rowof[pp < m ? pp : 0];
+          piece[q] = *reinterpret_cast<const vec_t*>(A + (int64_t)ph * n + cb + (t & 7) * EPV);
+        }
+      };
+      load_chunk(0);
+#pragma unroll
+      for (int s = 0; s < R; ++s) {
+        if (s > 0) __syncthreads();  // the previous chunk's rows are read
+#pragma unroll
+        for (int q = 0; q < 8; ++q) *reinterpret_cast<vec_t*>(&stage[(t >> 3) + 64 * q][(t & 7) * EPV]) = piece[q];
+        if (s + 1 < R) load_chunk(s + 1);
+        __syncthreads();
+#pragma unroll
+        for (int c = 0; c < IB; c += EPV) {
+          const vec_t x = *reinterpret_cast<const vec_t*>(&stage[t][c]);
+#pragma unroll
+          for (int e = 0; e < EPV; ++e) v[s][c + e] = x[e];
+        }
+      }
+    } else {
+#pragma unroll
+      for (int s = 0; s < R; ++s) {
+        const T* src = A + (int64_t)phys[s] * n + cb;
+#pragma unroll
+        for (int c = 0; c < IB; ++c) v[s][c] = (FULL || c < ib) ? src[c] : T(0);
+      }
     }
     {
       // U rows of the block (loads of every pass issued before any use)
@@ -258,16 +308,40 @@ __global__ __launch_bounds__(NTH) void leaf_kernel(LeafArgs<T> a) {
       }
     }
     // ---- the block back to its rows' physical places
+    if (STG && FULL && staged) {
 #pragma unroll
-    for (int s = 0; s < R; ++s) {
-      if (pos[s] < m) {
-        T* row = A + (int64_t)phys[s] * n + cb;
+      for (int s = 0; s < R; ++s) {
+        if (s > 0) __syncthreads();  // the previous chunk's pieces are stored
+        stage_phys[t] = pos[s] < m ? phys[s] : -1;
 #pragma unroll
-        for (int c = 0; c < IB; ++c)
-          if (c < ib) row[c] = v[s][c];
+        for (int c = 0; c < IB; c += EPV) {
+          vec_t x;
+#pragma unroll
+          for (int e = 0; e < EPV; ++e) x[e] = v[s][c + e];
+          *reinterpret_cast<vec_t*>(&stage[t][c]) = x;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+          const int lr = (t >> 3) + 64 * q;
+          const int ph = stage_phys[lr];
+          if (ph >= 0)
+            *reinterpret_cast<vec_t*>(A + (int64_t)ph * n + cb + (t & 7) * EPV) =
+                *reinterpret_cast<const vec_t*>(&stage[lr][(t & 7) * EPV]);
+        }
+      }
+    } else {
+#pragma unroll
+      for (int s = 0; s < R; ++s) {
+        if (pos[s] < m) {
+          T* row = A + (int64_t)phys[s] * n + cb;
+#pragma unroll
+          for (int c = 0; c < IB; ++c)
+            if (c < ib) row[c] = v[s][c];
+        }
       }
     }
-    __syncthreads();  // every read of rowof for this block is done
+    __syncthreads();  // every read of rowof (and of the staging rows) for this block is done
 #pragma unroll
     for (int s = 0; s < R; ++s)
       if (pos[s] < m) rowof[pos[s]] = phys[s];
