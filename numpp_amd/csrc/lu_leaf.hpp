@@ -220,12 +220,16 @@ __global__ __launch_bounds__(NTH) void leaf_kernel(LeafArgs<T> a) {
       const int buf = c & 1;
       T bv = T(-1);
       int bp = 1 << 28, bs = -1;
+      // (selects, no branches: a row outside [col, m) has key -1, which
+      // never wins, and neither does a NaN below position col)
 #pragma unroll
       for (int s = 0; s < R; ++s) {
-        if (pos[s] >= col && pos[s] < m) {
-          const T av = piv_key(v[s][c], pos[s], col);
-          if (av >= T(0) && piv_better(av, pos[s], bv, bp)) { bv = av; bp = pos[s]; bs = s; }
-        }
+        const bool el = pos[s] >= col && pos[s] < m;
+        const T av = el ? piv_key(v[s][c], pos[s], col) : T(-1);
+        const bool tk = av >= T(0) && piv_better(av, pos[s], bv, bp);
+        bv = tk ? av : bv;
+        bp = tk ? pos[s] : bp;
+        bs = tk ? s : bs;
       }
       T wv = bv;
       int wp = bp;
