@@ -81,9 +81,17 @@ __global__ __launch_bounds__(NTH) void split_rows_kernel(int64_t M, int K, int K
   const bool live = row < M;
   const int64_t lrow = live ? row : 0;
   const float* ar = A + (maps.a ? (int64_t)maps.a[bz * maps.stride + lrow] : lrow) * lda;
+  // 16-byte loads when every row starts on a 16-byte boundary
+  const bool vec = ((reinterpret_cast<uintptr_t>(Ab) | (uintptr_t)(lda * 4) | (uintptr_t)(sA * 4)) & 15) == 0;
   float amax = 0.f;
-  if (live)
-    for (int k = q; k < K; k += 4) amax = fmaxf(amax, fabsf(ar[k]));
+  if (live) {
+    const int K4 = vec ? (K & ~3) : 0;
+    for (int k = 4 * q; k < K4; k += 16) {
+      const float4 x = *reinterpret_cast<const float4*>(ar + k);
+      amax = fmaxf(amax, fmaxf(fmaxf(fabsf(x.x), fabsf(x.y)), fmaxf(fabsf(x.z), fabsf(x.w))));
+    }
+    for (int k = K4 + q; k < K; k += 4) amax = fmaxf(amax, fabsf(ar[k]));
+  }
   amax = block_max(amax, red);
   const int e = panel_exp(amax);
   if (t == 0) eA[bz * (Mp / SPAN) + panel] = e;
@@ -91,10 +99,20 @@ __global__ __launch_bounds__(NTH) void split_rows_kernel(int64_t M, int K, int K
   // thread q writes k = 8 * (q + 4 j) .. + 7 as 16-byte vectors
   for (int k0 = 8 * q; k0 < Kp; k0 += 32) {
     bf16x8 v[3];
+    float a8[8];
+    if (vec && k0 + 8 <= K) {
+      const float4 x0 = *reinterpret_cast<const float4*>(ar + k0);
+      const float4 x1 = *reinterpret_cast<const float4*>(ar + k0 + 4);
+      a8[0] = x0.x; a8[1] = x0.y; a8[2] = x0.z; a8[3] = x0.w;
+      a8[4] = x1.x; a8[5] = x1.y; a8[6] = x1.z; a8[7] = x1.w;
+    } else {
+#pragma unroll
+      for (int u = 0; u < 8; ++u) a8[u] = ar[min(k0 + u, K - 1)];  // clamped: padding reads a real entry
+    }
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
       const int k = k0 + u;
-      const float a = ar[min(k, K - 1)];  // clamped: padding reads a real entry
+      const float a = a8[u];
       const float x = (live && k < K) ? a * sc : 0.f;
       __bf16 o[3];
       split_val<NP>(x, o);
@@ -116,12 +134,21 @@ __global__ __launch_bounds__(NTH) void split_cols_kernel(int64_t N, int K, int K
                                                          int* __restrict__ eB, RowMaps maps) {
   __shared__ float red[4];
   __shared__ float tile[SPAN][SPAN + 1];  // [k][col]
+  // the B rows' physical indices, read once into LDS (K <= MAXK), so the
+  // row loads below wait for an LDS read rather than a global one
+  constexpr int MAXK = 2048;
+  __shared__ int rows_s[MAXK];
   const int64_t bz = blockIdx.y;
   const int panel = blockIdx.x;
   const float* B = Bb + bz * sB;
   const int32_t* mapb = maps.b ? maps.b + bz * maps.stride : nullptr;
-  auto brow = [&](int k) { return (int64_t)(mapb ? mapb[k] : k) * ldb; };
+  const bool staged = mapb && K <= MAXK;
   const int t = threadIdx.x;
+  if (staged) {
+    for (int k = t; k < K; k += NTH) rows_s[k] = mapb[k];
+    __syncthreads();
+  }
+  auto brow = [&](int k) { return (int64_t)(staged ? rows_s[k] : (mapb ? mapb[k] : k)) * ldb; };
   const int c = t & 63, kr = t >> 6;  // read: column c, rows kr, kr + 4, ...
   const int64_t col = (int64_t)panel * SPAN + c;
   const bool live = col < N;
@@ -172,8 +199,10 @@ __global__ __launch_bounds__(NTH) void bf16p_gemm_kernel(int64_t M, int64_t N, i
                                                          const int* __restrict__ eB, float* __restrict__ Cb,
                                                          int64_t ldc, int64_t sC, int tiles_m, int tiles_n,
                                                          RowMaps maps) {
-  __shared__ __bf16 As[2][NP][TM][KPAD];
-  __shared__ __bf16 Bs[2][NP][TN][KPAD];
+  // one LDS slice (NP = 3: 61 KB, so two workgroups share a CU and one's
+  // MFMAs cover the other's global-load latency)
+  __shared__ __bf16 As[NP][TM][KPAD];
+  __shared__ __bf16 Bs[NP][TN][KPAD];
   // Work order: workgroups are dispatched round-robin over the 8 XCDs (each
   // with its own L2); the permutation below gives XCD x a contiguous eighth of
   // the (matrix, tile) sequence, so whole matrices stay on one XCD, and each
@@ -213,13 +242,13 @@ __global__ __launch_bounds__(NTH) void bf16p_gemm_kernel(int64_t M, int64_t N, i
         rb[p][h] = *reinterpret_cast<const bf16x8*>(pb + p * plB + k0 + 8 * h);
       }
   };
-  auto store = [&](int buf) {
+  auto store = [&]() {
 #pragma unroll
     for (int p = 0; p < NP; ++p)
 #pragma unroll
       for (int h = 0; h < 2; ++h) {
-        *reinterpret_cast<bf16x8*>(&As[buf][p][sr][sk + 8 * h]) = ra[p][h];
-        *reinterpret_cast<bf16x8*>(&Bs[buf][p][sr][sk + 8 * h]) = rb[p][h];
+        *reinterpret_cast<bf16x8*>(&As[p][sr][sk + 8 * h]) = ra[p][h];
+        *reinterpret_cast<bf16x8*>(&Bs[p][sr][sk + 8 * h]) = rb[p][h];
       }
   };
 
@@ -229,20 +258,22 @@ __global__ __launch_bounds__(NTH) void bf16p_gemm_kernel(int64_t M, int64_t N, i
 #pragma unroll
     for (int j = 0; j < 4; ++j) acc[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
 
+  // per slice: the next slice's global loads go out first, the fragments are
+  // read from LDS and multiplied, then (after a barrier: every wave holds its
+  // fragments) the next slice is stored over the current one
   const int nk = Kp / BK;
   load(0);
-  store(0);
+  store();
   __syncthreads();
   for (int kt = 0; kt < nk; ++kt) {
-    const int cur = kt & 1;
     if (kt + 1 < nk) load((kt + 1) * BK);
     bf16x8 a[NP][4], b[NP][4];
 #pragma unroll
     for (int p = 0; p < NP; ++p) {
 #pragma unroll
-      for (int i = 0; i < 4; ++i) a[p][i] = *reinterpret_cast<const bf16x8*>(&As[cur][p][wm * 64 + i * 16 + li][lk]);
+      for (int i = 0; i < 4; ++i) a[p][i] = *reinterpret_cast<const bf16x8*>(&As[p][wm * 64 + i * 16 + li][lk]);
 #pragma unroll
-      for (int j = 0; j < 4; ++j) b[p][j] = *reinterpret_cast<const bf16x8*>(&Bs[cur][p][wn * 64 + j * 16 + li][lk]);
+      for (int j = 0; j < 4; ++j) b[p][j] = *reinterpret_cast<const bf16x8*>(&Bs[p][wn * 64 + j * 16 + li][lk]);
     }
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
@@ -262,7 +293,10 @@ __global__ __launch_bounds__(NTH) void bf16p_gemm_kernel(int64_t M, int64_t N, i
         acc[i][j] = c;
       }
     }
-    if (kt + 1 < nk) store(cur ^ 1);
+    if (kt + 1 < nk) {
+      __syncthreads();
+      store();
+    }
     __syncthreads();
   }
 
