@@ -226,8 +226,10 @@ __global__ __launch_bounds__(NTH) void bf16p_gemm_kernel(int64_t M, int64_t N, i
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int wm = wave >> 1, wn = wave & 1;
   const int li = lane & 15, lk = (lane >> 4) * 8;
-  // staging: thread t copies row (t >> 1), k half (t & 1) * 16 of each plane
-  const int sr = t >> 1, sk = (t & 1) * 16;
+  // staging: thread t copies the 16-byte k chunk t % 4 of rows t / 4 and
+  // t / 4 + 64 of each plane (four neighbouring lanes read a row's 64-byte
+  // slice in one piece)
+  const int sr = t >> 2, sk = (t & 3) * 8;
   const __bf16* pa = PA + (bz * NP * Mp + m0 + sr) * Kp + sk;
   const __bf16* pb = PB + (bz * NP * Np + n0 + sr) * Kp + sk;
   const int64_t plA = Mp * Kp, plB = Np * Kp;
@@ -238,8 +240,8 @@ __global__ __launch_bounds__(NTH) void bf16p_gemm_kernel(int64_t M, int64_t N, i
     for (int p = 0; p < NP; ++p)
 #pragma unroll
       for (int h = 0; h < 2; ++h) {
-        ra[p][h] = *reinterpret_cast<const bf16x8*>(pa + p * plA + k0 + 8 * h);
-        rb[p][h] = *reinterpret_cast<const bf16x8*>(pb + p * plB + k0 + 8 * h);
+        ra[p][h] = *reinterpret_cast<const bf16x8*>(pa + p * plA + 64 * h * (int64_t)Kp + k0);
+        rb[p][h] = *reinterpret_cast<const bf16x8*>(pb + p * plB + 64 * h * (int64_t)Kp + k0);
       }
   };
   auto store = [&]() {
@@ -247,8 +249,8 @@ __global__ __launch_bounds__(NTH) void bf16p_gemm_kernel(int64_t M, int64_t N, i
     for (int p = 0; p < NP; ++p)
 #pragma unroll
       for (int h = 0; h < 2; ++h) {
-        *reinterpret_cast<bf16x8*>(&As[p][sr][sk + 8 * h]) = ra[p][h];
-        *reinterpret_cast<bf16x8*>(&Bs[p][sr][sk + 8 * h]) = rb[p][h];
+        *reinterpret_cast<bf16x8*>(&As[p][sr + 64 * h][sk]) = ra[p][h];
+        *reinterpret_cast<bf16x8*>(&Bs[p][sr + 64 * h][sk]) = rb[p][h];
       }
   };
 
