@@ -11,7 +11,7 @@
 //     stored K-contiguous (A row-major [row][k], B transposed [col][k]), K padded
 //     with zeros to a multiple of 32 and rows/columns to a multiple of 128;
 //  2. product: a plain BF16 MFMA GEMM over the planes, 128 x 128 tiles, 32-deep
-//     K slices double-buffered through LDS, the partial products of one slice
+//     K slices through one XOR-swizzled LDS slice, the partial products of one slice
 //     accumulated smallest first into one FP32 accumulator per output; the
 //     epilogue unscales by 2^(eA + eB) of the output's panels and subtracts.
 // terms = 1 keeps only h (plain BF16); 3 uses two pieces (h, m) and the
@@ -31,7 +31,11 @@ namespace npph {
 namespace {
 
 constexpr int SPAN = 64;       // panel: rows of A / columns of B sharing one scale
-constexpr int TM = 128, TN = 128, BK = 32, KPAD = BK + 8;  // KPAD: LDS row (bf16)
+constexpr int TM = 128, TN = 128, BK = 32;
+// LDS rows are BK bf16 = four 16-byte chunks, unpadded; chunk c of row r is
+// stored at c ^ ((r >> 2) & 3), so the 16 rows one fragment read covers and
+// the rows a staging write covers fall in distinct banks.
+__device__ __forceinline__ int swz(int r, int c) { return (c ^ ((r >> 2) & 3)) * 8; }
 constexpr int NTH = 256;
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
@@ -201,8 +205,8 @@ __global__ __launch_bounds__(NTH) void bf16p_gemm_kernel(int64_t M, int64_t N, i
                                                          RowMaps maps) {
   // one LDS slice (NP = 3: 61 KB, so two workgroups share a CU and one's
   // MFMAs cover the other's global-load latency)
-  __shared__ __bf16 As[NP][TM][KPAD];
-  __shared__ __bf16 Bs[NP][TN][KPAD];
+  __shared__ __bf16 As[NP][TM][BK];
+  __shared__ __bf16 Bs[NP][TN][BK];
   // Work order: workgroups are dispatched round-robin over the 8 XCDs (each
   // with its own L2); the permutation below gives XCD x a contiguous eighth of
   // the (matrix, tile) sequence, so whole matrices stay on one XCD, and each
@@ -249,8 +253,8 @@ __global__ __launch_bounds__(NTH) void bf16p_gemm_kernel(int64_t M, int64_t N, i
     for (int p = 0; p < NP; ++p)
 #pragma unroll
       for (int h = 0; h < 2; ++h) {
-        *reinterpret_cast<bf16x8*>(&As[p][sr + 64 * h][sk]) = ra[p][h];
-        *reinterpret_cast<bf16x8*>(&Bs[p][sr + 64 * h][sk]) = rb[p][h];
+        *reinterpret_cast<bf16x8*>(&As[p][sr + 64 * h][swz(sr + 64 * h, sk >> 3)]) = ra[p][h];
+        *reinterpret_cast<bf16x8*>(&Bs[p][sr + 64 * h][swz(sr + 64 * h, sk >> 3)]) = rb[p][h];
       }
   };
 
@@ -273,9 +277,15 @@ __global__ __launch_bounds__(NTH) void bf16p_gemm_kernel(int64_t M, int64_t N, i
 #pragma unroll
     for (int p = 0; p < NP; ++p) {
 #pragma unroll
-      for (int i = 0; i < 4; ++i) a[p][i] = *reinterpret_cast<const bf16x8*>(&As[p][wm * 64 + i * 16 + li][lk]);
+      for (int i = 0; i < 4; ++i) {
+        const int r = wm * 64 + i * 16 + li;
+        a[p][i] = *reinterpret_cast<const bf16x8*>(&As[p][r][swz(r, lk >> 3)]);
+      }
 #pragma unroll
-      for (int j = 0; j < 4; ++j) b[p][j] = *reinterpret_cast<const bf16x8*>(&Bs[p][wn * 64 + j * 16 + li][lk]);
+      for (int j = 0; j < 4; ++j) {
+        const int r = wn * 64 + j * 16 + li;
+        b[p][j] = *reinterpret_cast<const bf16x8*>(&Bs[p][r][swz(r, lk >> 3)]);
+      }
     }
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
