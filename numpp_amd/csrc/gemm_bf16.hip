@@ -203,7 +203,7 @@ __global__ __launch_bounds__(NTH) void bf16p_gemm_kernel(int64_t M, int64_t N, i
                                                          const int* __restrict__ eB, float* __restrict__ Cb,
                                                          int64_t ldc, int64_t sC, int tiles_m, int tiles_n,
                                                          RowMaps maps) {
-  // one LDS slice (NP = 3: 61 KB, so two workgroups share a CU and one's
+  // one LDS slice (NP = 3: 49 KB, so two workgroups share a CU and one's
   // MFMAs cover the other's global-load latency)
   __shared__ __bf16 As[NP][TM][BK];
   __shared__ __bf16 Bs[NP][TN][BK];
