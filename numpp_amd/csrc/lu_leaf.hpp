@@ -52,6 +52,7 @@ __global__ __launch_bounds__(NTH) void leaf_kernel(LeafArgs<T> a) {
   int32_t* ipiv = a.ipiv + mat * n;
   int32_t* perm = a.perm + mat * n;
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const unsigned lt = threadIdx.x;  // the thread index, unsigned for cheap / and %
   constexpr int QC = (R >= 4 && sizeof(T) == 8) ? 4 : 8;  // L columns per update chunk
   static_assert(IB % QC == 0 && LEAF % IB == 0, "inner blocks must hold whole update chunks");
 
@@ -68,18 +69,22 @@ __global__ __launch_bounds__(NTH) void leaf_kernel(LeafArgs<T> a) {
   // a global store inside the column loop would make every column barrier
   // wait for it (the barrier's release fence)
   __shared__ int piv_s[LEAF];
-  // Staging of the block's rows for panels whose inner-block rows are 128
-  // bytes (FP32 IB = 32, FP64 IB = 16) with 512 threads: the gather and the
-  // write-back move each row as eight 16-byte pieces read by eight
-  // neighbouring lanes (64 rows per wave instruction, whole cache lines),
-  // passing through LDS in 512-row chunks to and from the one-row-per-lane
-  // register layout. With one row per lane each wave instruction would touch
-  // 64 cache lines for 16 bytes apiece, which one CU's memory pipe serves at a
-  // third of the rate. Rows are padded to 144 bytes so the lanes' 16-byte LDS
-  // reads start 36 banks apart.
-  constexpr bool STG = NTH == 512 && IB * (int)sizeof(T) == 128;
+  // Staging of the block's rows for panels of 512 threads whose inner-block
+  // rows are 128 bytes (FP32 IB = 32, FP64 IB = 16) or 256 bytes (FP32
+  // IB = 64): the gather and the write-back move each row as 16-byte pieces
+  // read by LPR neighbouring lanes (whole cache lines), passing through LDS in
+  // 512-row chunks (72 or 136 KB) to and from the one-row-per-lane register
+  // layout. With one row per lane each wave instruction would touch 64 cache
+  // lines for 16 bytes apiece, which one CU's memory pipe serves at a third of
+  // the rate. Rows are padded by 16 bytes so the lanes' 16-byte LDS reads
+  // start 4 banks apart. A thread moves its LPR pieces of a chunk in batches
+  // of eight, one batch in flight at a time.
+  constexpr int RB = IB * (int)sizeof(T);   // bytes per inner-block row
+  constexpr bool STG = NTH == 512 && (RB == 128 || (RB == 256 && sizeof(T) == 4));
   constexpr int EPV = 16 / (int)sizeof(T);  // elements per 16-byte piece
   constexpr int SW = IB + EPV;              // staging row stride (elements)
+  constexpr int LPR = RB / 16;              // lanes per row (= pieces per thread)
+  constexpr int NBT = STG ? LPR / 8 : 1;    // batches of eight pieces per chunk
   typedef T vec_t __attribute__((ext_vector_type(16 / sizeof(T))));
   __shared__ __attribute__((aligned(16))) T stage[STG ? NTH : 1][STG ? SW : 1];
   __shared__ int stage_phys[STG ? NTH : 1];
@@ -109,30 +114,37 @@ __global__ __launch_bounds__(NTH) void leaf_kernel(LeafArgs<T> a) {
       phys[s] = rowof[p0 < m ? p0 : 0];
     }
     if (STG && FULL && staged) {
-      // piece t % 8 of rows t / 8 + 64 q of chunk s (the next chunk's loads
-      // are in flight while the current one passes through LDS)
+      // batch g = (chunk s, batch b): piece lt % LPR of the chunk's rows
+      // lt / LPR + (NTH / LPR) q for q in 8 b .. 8 b + 7 (the next batch's
+      // loads are in flight while the current one is stored to LDS and, after
+      // a chunk's last batch, while the chunk is read back)
       vec_t piece[8];
-      auto load_chunk = [&](int s) __attribute__((always_inline)) {
+      auto load_batch = [&](int g) __attribute__((always_inline)) {
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
-          const int pp = jb + NTH * s + (t >> 3) + 64 * q;
+          const int pp = jb + NTH * (g / NBT) + lt / LPR + (NTH / LPR) * (q + 8 * (g % NBT));
           const int ph = rowof[pp < m ? pp : 0];
-          piece[q] = *reinterpret_cast<const vec_t*>(A + (int64_t)ph * n + cb + (t & 7) * EPV);
+          piece[q] = *reinterpret_cast<const vec_t*>(A + (int64_t)ph * n + cb + (lt % LPR) * EPV);
         }
       };
-      load_chunk(0);
+      load_batch(0);
 #pragma unroll
-      for (int s = 0; s < R; ++s) {
-        if (s > 0) __syncthreads();  // the previous chunk's rows are read
+      for (int g = 0; g < R * NBT; ++g) {
+        if (g > 0 && g % NBT == 0) __syncthreads();  // the previous chunk's rows are read
 #pragma unroll
-        for (int q = 0; q < 8; ++q) *reinterpret_cast<vec_t*>(&stage[(t >> 3) + 64 * q][(t & 7) * EPV]) = piece[q];
-        if (s + 1 < R) load_chunk(s + 1);
-        __syncthreads();
+        for (int q = 0; q < 8; ++q)
+          *reinterpret_cast<vec_t*>(&stage[lt / LPR + (NTH / LPR) * (q + 8 * (g % NBT))][(lt % LPR) * EPV]) =
+              piece[q];
+        if (g + 1 < R * NBT) load_batch(g + 1);
+        if (g % NBT == NBT - 1) {
+          __syncthreads();
+          const int s = g / NBT;
 #pragma unroll
-        for (int c = 0; c < IB; c += EPV) {
-          const vec_t x = *reinterpret_cast<const vec_t*>(&stage[t][c]);
+          for (int c = 0; c < IB; c += EPV) {
+            const vec_t x = *reinterpret_cast<const vec_t*>(&stage[t][c]);
 #pragma unroll
-          for (int e = 0; e < EPV; ++e) v[s][c + e] = x[e];
+            for (int e = 0; e < EPV; ++e) v[s][c + e] = x[e];
+          }
         }
       }
     } else {
@@ -326,12 +338,12 @@ __global__ __launch_bounds__(NTH) void leaf_kernel(LeafArgs<T> a) {
         }
         __syncthreads();
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {
-          const int lr = (t >> 3) + 64 * q;
+        for (int q = 0; q < LPR; ++q) {
+          const int lr = lt / LPR + (NTH / LPR) * q;
           const int ph = stage_phys[lr];
           if (ph >= 0)
-            *reinterpret_cast<vec_t*>(A + (int64_t)ph * n + cb + (t & 7) * EPV) =
-                *reinterpret_cast<const vec_t*>(&stage[lr][(t & 7) * EPV]);
+            *reinterpret_cast<vec_t*>(A + (int64_t)ph * n + cb + (lt % LPR) * EPV) =
+                *reinterpret_cast<const vec_t*>(&stage[lr][(lt % LPR) * EPV]);
         }
       }
     } else {
