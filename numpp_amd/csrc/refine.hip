@@ -27,44 +27,60 @@ __device__ __forceinline__ void atomic_max_nonneg(double* addr, double v) {
   atomicMax(reinterpret_cast<unsigned long long*>(addr), (unsigned long long)__double_as_longlong(v));
 }
 
-// A (FP64) -> Alo (FP32) and ||A||_inf per matrix. One wave per row.
-__global__ __launch_bounds__(256) void demote_kernel(int64_t count, int n, const double* A, float* Alo,
+// A (FP64) -> Alo (FP32, or an FP64 copy) and ||A||_inf per matrix. Each wave
+// converts kDemoteRows consecutive rows, eight 16-byte loads per lane in
+// flight, and folds their absolute row sums into one atomic maximum per
+// matrix it touches (the atomics are few: on one address they serialise).
+constexpr int kDemoteRows = 4;
+template <class To>
+__global__ __launch_bounds__(256) void demote_kernel(int64_t count, int n, const double* A, To* Alo,
                                                      double* anorm) {
-  const int64_t gw = ((int64_t)blockIdx.x * 256 + threadIdx.x) >> 6;
+  typedef To pair_t __attribute__((ext_vector_type(2)));
+  const int64_t row0 = (((int64_t)blockIdx.x * 256 + threadIdx.x) >> 6) * kDemoteRows;
   const int lane = threadIdx.x & 63;
-  if (gw >= count * n) return;
-  const int64_t mat = gw / n;
-  const double* a = A + gw * n;
-  float* o = Alo + gw * n;
-  double s = 0.0;
-  if ((n & 1) == 0) {
-    int j = 2 * lane;
-    // four 16-byte loads in flight per lane before any is used
-    for (; j + 384 < n; j += 512) {
-      double2 v[4];
+  const int64_t rows = count * n;
+  int64_t cur = -1;  // matrix of the running maximum
+  double best = 0.0;
+  for (int r = 0; r < kDemoteRows && row0 + r < rows; ++r) {
+    const int64_t row = row0 + r;
+    const int64_t mat = row / n;
+    const double* a = A + row * n;
+    To* o = Alo + row * n;
+    double s = 0.0;
+    if ((n & 1) == 0) {
+      int j = 2 * lane;
+      for (; j + 896 < n; j += 1024) {
+        double2 v[8];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) v[u] = *reinterpret_cast<const double2*>(a + j + 128 * u);
+        for (int u = 0; u < 8; ++u) v[u] = *reinterpret_cast<const double2*>(a + j + 128 * u);
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        *reinterpret_cast<float2*>(o + j + 128 * u) = make_float2((float)v[u].x, (float)v[u].y);
-        s += fabs(v[u].x) + fabs(v[u].y);
+        for (int u = 0; u < 8; ++u) {
+          *reinterpret_cast<pair_t*>(o + j + 128 * u) = pair_t{(To)v[u].x, (To)v[u].y};
+          s += fabs(v[u].x) + fabs(v[u].y);
+        }
+      }
+      for (; j < n; j += 128) {
+        const double2 v = *reinterpret_cast<const double2*>(a + j);
+        *reinterpret_cast<pair_t*>(o + j) = pair_t{(To)v.x, (To)v.y};
+        s += fabs(v.x) + fabs(v.y);
+      }
+    } else {
+      for (int j = lane; j < n; j += 64) {
+        const double v = a[j];
+        o[j] = (To)v;
+        s += fabs(v);
       }
     }
-    for (; j < n; j += 128) {
-      const double2 v = *reinterpret_cast<const double2*>(a + j);
-      *reinterpret_cast<float2*>(o + j) = make_float2((float)v.x, (float)v.y);
-      s += fabs(v.x) + fabs(v.y);
-    }
-  } else {
-    for (int j = lane; j < n; j += 64) {
-      const double v = a[j];
-      o[j] = (float)v;
-      s += fabs(v);
-    }
-  }
 #pragma unroll
-  for (int m = 32; m >= 1; m >>= 1) s += __shfl_xor(s, m, 64);
-  if (lane == 0) atomic_max_nonneg(anorm + mat, s);
+    for (int m = 32; m >= 1; m >>= 1) s += __shfl_xor(s, m, 64);
+    if (mat != cur) {
+      if (cur >= 0 && lane == 0) atomic_max_nonneg(anorm + cur, best);
+      cur = mat;
+      best = 0.0;
+    }
+    best = fmax(best, s);
+  }
+  if (cur >= 0 && lane == 0) atomic_max_nonneg(anorm + cur, best);
 }
 
 // ||b_j||_inf per (matrix, column): one workgroup per matrix, CPW columns
@@ -93,26 +109,6 @@ __global__ __launch_bounds__(256) void bnorm_kernel(int n, int nrhs, const doubl
       bmax[mat * nrhs + c0 + t] = fmax(fmax(red[0][t], red[1][t]), fmax(red[2][t], red[3][t]));
     __syncthreads();
   }
-}
-
-// ||A||_inf per matrix only (plain FP64 path, for the reported eta).
-__global__ __launch_bounds__(256) void rowabs_kernel(int64_t count, int n, const double* A, double* anorm) {
-  const int64_t gw = ((int64_t)blockIdx.x * 256 + threadIdx.x) >> 6;
-  const int lane = threadIdx.x & 63;
-  if (gw >= count * n) return;
-  const double* a = A + gw * n;
-  double s = 0.0;
-  if ((n & 1) == 0) {
-    for (int j = 2 * lane; j < n; j += 128) {
-      const double2 v = *reinterpret_cast<const double2*>(a + j);
-      s += fabs(v.x) + fabs(v.y);
-    }
-  } else {
-    for (int j = lane; j < n; j += 64) s += fabs(a[j]);
-  }
-#pragma unroll
-  for (int m = 32; m >= 1; m >>= 1) s += __shfl_xor(s, m, 64);
-  if (lane == 0) atomic_max_nonneg(anorm + gw / n, s);
 }
 
 __global__ void zero_int_kernel(int32_t* p, int n) {
@@ -700,8 +696,8 @@ int refine_factor(const npph_refine_opts& o, const double* A, RefineFactors& f, 
   if (o.precision == NPPH_PREC_FP64) {
     const int64_t per_a = (int64_t)n * n;
     if (!prepped) {
-      hipLaunchKernelGGL(rowabs_kernel, g1(count * n * 64), dim3(256), 0, s, count, n, A, f.anorm);
-      NPPH_HIP(hipMemcpyAsync(f.lu64, A, sizeof(double) * count * per_a, hipMemcpyDeviceToDevice, s));
+      hipLaunchKernelGGL(demote_kernel<double>, g1(ceil_div(count * n, kDemoteRows) * 64), dim3(256), 0, s, count,
+                         n, A, f.lu64, f.anorm);
     }
     if (n <= 64) {
       UniformBatch u{count, n, f.lu64, per_a, f.lu64, f.ipiv, f.info, 0, nullptr, 0, nullptr};
@@ -718,7 +714,7 @@ int refine_factor(const npph_refine_opts& o, const double* A, RefineFactors& f, 
     return NPPH_ERR_INVALID;
   }
   const int terms = o.precision == NPPH_PREC_BF16 ? (o.bf16_terms > 0 ? o.bf16_terms : 6) : 0;
-  if (!prepped) hipLaunchKernelGGL(demote_kernel, g1(count * n * 64), dim3(256), 0, s, count, n, A, f.lu32, f.anorm);
+  if (!prepped) hipLaunchKernelGGL(demote_kernel<float>, g1(ceil_div(count * n, kDemoteRows) * 64), dim3(256), 0, s, count, n, A, f.lu32, f.anorm);
   NPPH_HIP(hipGetLastError());
   return blocked_getrf_perm<float>(count, n, f.lu32, (int64_t)n * n, f.ipiv, f.perm, f.info, terms, nullptr, s);
 }
