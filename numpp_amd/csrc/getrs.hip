@@ -34,6 +34,7 @@ struct RefIO {
   const double* scale = nullptr;  // per (system, column) power-of-two scale
   double* X = nullptr;            // iterate: X (+)= correction / scale
   int first = 0;                  // 1: X = correction / scale
+  const int32_t* live = nullptr;  // device count of systems still iterating
 };
 
 __device__ __forceinline__ v4f mfma16(float a, float b, v4f c) {
@@ -446,7 +447,7 @@ int getrs_impl(int64_t count, int n, int nrhs, const T* LU, int64_t strideA, con
     ProfScope ps(kProfGetrs, 2.0 * count * n * n * nrhs,
                  (double)count * (sizeof(T) * (double)n * n + 3.0 * sizeof(T) * n * nrhs), s);
     return getrs_mw<T, REF>(count, n, nrhs, LU, strideA, perm, rowmap, B, strideB, X, strideX, info, mask, ref.R,
-                            ref.scale, ref.X, ref.first, s);
+                            ref.scale, ref.X, ref.first, ref.live, s);
   }
   const int cw = chunk_width<T>(count, n, nrhs);
   const int nz = (int)ceil_div(nrhs, cw);
@@ -480,10 +481,10 @@ int blocked_getrs_perm(int64_t count, int n, int nrhs, const T* LU, int64_t stri
 
 int getrs_refine(int64_t count, int n, int nrhs, const float* LU, int64_t strideA, const int32_t* perm,
                  const double* R, const double* scale, float* Y, double* X, int first, const int32_t* mask,
-                 hipStream_t s, const int32_t* rowmap) {
+                 hipStream_t s, const int32_t* rowmap, const int32_t* live) {
   const int64_t per = (int64_t)n * nrhs;
   return getrs_impl<float, true>(count, n, nrhs, LU, strideA, perm, rowmap, nullptr, per, Y, per, nullptr, mask,
-                                 RefIO{R, scale, X, first}, s);
+                                 RefIO{R, scale, X, first, live}, s);
 }
 
 template int blocked_getrs_perm<double>(int64_t, int, int, const double*, int64_t, const int32_t*, const double*,

@@ -52,6 +52,7 @@ struct MwRef {
   const double* scale = nullptr;  // per (system, column) scale of the residual
   double* X = nullptr;            // FP64 iterate: X (+)= correction / scale
   int first = 0;
+  const int32_t* live = nullptr;  // device count of systems still iterating
 };
 
 __device__ __forceinline__ v4f mfma_mw(float a, float b, v4f c) {
@@ -122,6 +123,9 @@ __global__ __launch_bounds__(MT) void sweep_mw_kernel(int64_t count, int n, int 
   __shared__ T part[MB][16];
   __shared__ uint32_t tk_s;
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  // a refinement sweep queued after the last system stopped: every workgroup
+  // leaves before taking a ticket (thousands of tickets on one counter)
+  if (REF && ref.live && *ref.live == 0) return;
   if (t == 0) tk_s = atomicAdd(&sync[LOWER ? 0 : 1], 1u);
   __syncthreads();
   const int64_t per = count * nz;
@@ -368,14 +372,14 @@ bool getrs_mw_wanted(int64_t count, int n, int nrhs) {
 template <class T, bool REF>
 int getrs_mw(int64_t count, int n, int nrhs, const T* LU, int64_t strideA, const int32_t* perm, const int32_t* rowmap,
              const T* B, int64_t strideB, T* X, int64_t strideX, const int32_t* info, const int32_t* mask,
-             const double* R, const double* scale, double* X64, int first, hipStream_t s) {
+             const double* R, const double* scale, double* X64, int first, const int32_t* live, hipStream_t s) {
   const int cw = nrhs <= 4 ? 4 : (nrhs <= 8 ? 8 : 16);
   const int nz = (int)ceil_div(nrhs, cw);
   const int64_t words = kSyncHead + 2 * count * nz * ceil_div(n, MB) * (MB * cw * (int64_t)(sizeof(T) / 4));
   uint32_t* sync = nullptr;
   NPPH_TRY(workspace(kWsSync, sizeof(uint32_t) * (size_t)words, (void**)&sync));
   hipLaunchKernelGGL(mw_clear_kernel, dim3((unsigned)ceil_div(words, 256)), dim3(256), 0, s, sync, words);
-  MwRef ref{R, scale, X64, first};
+  MwRef ref{R, scale, X64, first, live};
   if (cw == 4)
     launch_mw<T, 4, REF>(count, n, nrhs, nz, LU, strideA, perm, rowmap, B, strideB, X, strideX, info, mask, ref, sync, s);
   else if (cw == 8)
@@ -389,12 +393,12 @@ int getrs_mw(int64_t count, int n, int nrhs, const T* LU, int64_t strideA, const
 
 template int getrs_mw<double, false>(int64_t, int, int, const double*, int64_t, const int32_t*, const int32_t*,
                                      const double*, int64_t, double*, int64_t, const int32_t*, const int32_t*,
-                                     const double*, const double*, double*, int, hipStream_t);
+                                     const double*, const double*, double*, int, const int32_t*, hipStream_t);
 template int getrs_mw<float, false>(int64_t, int, int, const float*, int64_t, const int32_t*, const int32_t*,
                                     const float*, int64_t, float*, int64_t, const int32_t*, const int32_t*,
-                                    const double*, const double*, double*, int, hipStream_t);
+                                    const double*, const double*, double*, int, const int32_t*, hipStream_t);
 template int getrs_mw<float, true>(int64_t, int, int, const float*, int64_t, const int32_t*, const int32_t*,
                                    const float*, int64_t, float*, int64_t, const int32_t*, const int32_t*,
-                                   const double*, const double*, double*, int, hipStream_t);
+                                   const double*, const double*, double*, int, const int32_t*, hipStream_t);
 
 }  // namespace npph

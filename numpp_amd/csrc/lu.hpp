@@ -76,10 +76,11 @@ int blocked_getrs_perm(int64_t count, int n, int nrhs, const T* LU, int64_t stri
 // One refinement correction with FP32 factors: D = U^-1 L^-1 P fp32(R * scale)
 // (per-column scale), then X = D / scale (first) or X += D / scale, in two
 // sweeps; Y (count x n x nrhs FP32) holds the forward sweep's result. Only
-// systems with mask == 1 (still iterating) are corrected.
+// systems with mask == 1 (still iterating) are corrected. live, when given,
+// is a device count of those systems: at zero the sweeps return at once.
 int getrs_refine(int64_t count, int n, int nrhs, const float* LU, int64_t strideA, const int32_t* perm,
                  const double* R, const double* scale, float* Y, double* X, int first, const int32_t* mask,
-                 hipStream_t s, const int32_t* rowmap);
+                 hipStream_t s, const int32_t* rowmap, const int32_t* live = nullptr);
 
 // Sweeps with many workgroups per system (getrs_mw.hip): wanted for stacks of
 // few systems of order >= 512, where one workgroup per system and column chunk
@@ -90,7 +91,7 @@ bool getrs_mw_wanted(int64_t count, int n, int nrhs);
 template <class T, bool REF>
 int getrs_mw(int64_t count, int n, int nrhs, const T* LU, int64_t strideA, const int32_t* perm, const int32_t* rowmap,
              const T* B, int64_t strideB, T* X, int64_t strideX, const int32_t* info, const int32_t* mask,
-             const double* R, const double* scale, double* X64, int first, hipStream_t s);
+             const double* R, const double* scale, double* X64, int first, const int32_t* live, hipStream_t s);
 
 // perm from LAPACK-style pivots (replays the swaps).
 int ipiv_to_perm(int64_t count, int n, const int32_t* ipiv, int32_t* perm, hipStream_t s);

@@ -100,8 +100,12 @@ __global__ __launch_bounds__(256) void bnorm_kernel(int n, int nrhs, const doubl
   for (int c0 = 0; c0 < nrhs; c0 += cpw) {
     const int c = c0 + cl;
     double m = 0.0;
-    if (c < nrhs)
+    // eight loads in flight per thread (one per pass would leave the pass
+    // loop bound by load latency: a workgroup per matrix is few workgroups)
+    if (c < nrhs) {
+#pragma unroll 8
       for (int i = r; i < n; i += rp) m = fmax(m, fabs(B[(int64_t)i * nrhs + c]));
+    }
     for (int o = cpw; o < 64; o <<= 1) m = fmax(m, __shfl_xor(m, o, 64));
     if (lane < cpw) red[wave][lane] = m;
     __syncthreads();
@@ -795,8 +799,9 @@ int corr_step(RefineRun& r, int it, hipStream_t s) {
     hipLaunchKernelGGL(zero_int_kernel, dim3(1), dim3(64), 0, s, r.ctr, 1);  // the next test's counter
     return NPPH_OK;
   }
+  // r.ctr[0]: systems still iterating after the last test (cleared below)
   NPPH_TRY(getrs_refine(r.count, r.n, r.nrhs, r.f.lu32, (int64_t)r.n * r.n, r.f.perm, r.R, r.scale, r.Dlo, r.X, 0,
-                        r.active, s, r.f.perm));
+                        r.active, s, r.f.perm, r.ctr));
   hipLaunchKernelGGL(count_sweep_kernel, g1(r.count), dim3(256), 0, s, r.count, r.active, r.iters, r.ctr);
   NPPH_HIP(hipGetLastError());
   return NPPH_OK;
