@@ -10,6 +10,7 @@
 // loop when every column meets tol; one that stagnates (eta not halving) or
 // reaches max_iter is re-solved with the plain FP64 LU when fallback is on.
 // Per-system arithmetic never depends on the other systems in the stack.
+#include <algorithm>
 #include <vector>
 
 #include "common.hpp"
@@ -180,8 +181,8 @@ __global__ void count_sweep_kernel(int64_t count, const int32_t* active, int32_t
 }
 
 // R = B - A X in two passes. Pass 1 (residual_partial_kernel): one
-// workgroup per (64-row block, matrix, 512-column chunk x 8-RHS chunk) stages
-// its X chunk in LDS once and walks its rows in four 16-row passes (4 rows per
+// workgroup per (row block, matrix, 512-column chunk x 8-RHS chunk) stages
+// its X chunk in LDS once and walks its rows in npass 16-row passes (4 rows per
 // wave, two adjacent columns per lane in each of four 128-column slices,
 // 16-byte loads); the first pass's A loads are issued before the X staging, so
 // that traffic is in flight while X arrives; its 32 partial sums per lane are
@@ -192,8 +193,8 @@ __global__ void count_sweep_kernel(int64_t count, const int32_t* active, int32_t
 // partials in chunk order (deterministic), forms r = b - sum and the per-column
 // max |r|. The block of matrix rows with index 0 also reports max |x| per column.
 constexpr int RES_RPW = 4, RES_ROWS = 4 * RES_RPW, RES_JC = 512, RES_NC = 8;
-constexpr int RES_PASS = 4;  // 16-row passes per workgroup over one staged X chunk
-__global__ __launch_bounds__(256) void residual_partial_kernel(int n, int nrhs, int nchunk, const double* Ab,
+constexpr int RES_PASS = 4;  // least 16-row passes per workgroup over one staged X chunk
+__global__ __launch_bounds__(256) void residual_partial_kernel(int n, int nrhs, int nchunk, int npass, const double* Ab,
                                                                const double* Xb, double* part, double* xmax,
                                                                const int32_t* active) {
   const int64_t mat = blockIdx.y;
@@ -232,7 +233,7 @@ __global__ __launch_bounds__(256) void residual_partial_kernel(int n, int nrhs, 
       }
     }
   };
-  const int rowbase = blockIdx.x * (RES_ROWS * RES_PASS) + wave * RES_RPW;
+  const int rowbase = blockIdx.x * (RES_ROWS * npass) + wave * RES_RPW;
   load_a(rowbase);
   double xm = 0.0;
 #pragma unroll
@@ -248,7 +249,7 @@ __global__ __launch_bounds__(256) void residual_partial_kernel(int n, int nrhs, 
     if (cc < nc) atomic_max_nonneg(xmax + mat * nrhs + c0 + cc, xm);
   }
   __syncthreads();
-  for (int pass = 0; pass < RES_PASS; ++pass) {
+  for (int pass = 0; pass < npass; ++pass) {
     const int row0 = rowbase + pass * RES_ROWS;
     if (pass > 0) {
       if (row0 - wave * RES_RPW >= n) break;  // uniform over the workgroup
@@ -649,9 +650,24 @@ int residual_launch(int64_t count, int n, int nrhs, const double* A, const doubl
   const int nchunk = (int)ceil_div(n, RES_JC), nz = (int)ceil_div(nrhs, RES_NC);
   double* part;
   NPPH_TRY(workspace(kWsPart, sizeof(double) * (size_t)count * nchunk * n * nrhs, (void**)&part));
-  dim3 grid((unsigned)ceil_div(n, RES_ROWS * RES_PASS), (unsigned)count, (unsigned)(nchunk * nz));
+  // rows per workgroup: enough that the grid is one wave of resident
+  // workgroups (at least RES_PASS passes each): long-lived workgroups stream
+  // A at a higher rate than many short ones, and a partial second wave of
+  // workgroups leaves most CUs idle at the end (64 x 2048: 1.40 -> 1.27 ms
+  // for three residuals)
+  static const int64_t slots = [] {
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, residual_partial_kernel, 256, 0) != hipSuccess ||
+        per_cu <= 0)
+      per_cu = 1;
+    return (int64_t)(cu_count() > 0 ? cu_count() : 256) * per_cu;
+  }();
+  const int64_t pairs = count * nchunk * nz;
+  const int64_t blocks_per_pair = std::max<int64_t>(1, slots / pairs);
+  const int npass = (int)std::max<int64_t>(RES_PASS, ceil_div(ceil_div(n, RES_ROWS), blocks_per_pair));
+  dim3 grid((unsigned)ceil_div(n, RES_ROWS * npass), (unsigned)count, (unsigned)(nchunk * nz));
   ProfScope ps(kProfResidual, 2.0 * count * n * n * nrhs, 8.0 * count * ((double)n * n + 3.0 * n * nrhs), s);
-  hipLaunchKernelGGL(residual_partial_kernel, grid, dim3(256), 0, s, n, nrhs, nchunk, A, X, part, xmax, active);
+  hipLaunchKernelGGL(residual_partial_kernel, grid, dim3(256), 0, s, n, nrhs, nchunk, npass, A, X, part, xmax, active);
   hipLaunchKernelGGL(residual_finish_kernel, g1(count * n * nrhs), dim3(256), 0, s, count, n, nrhs, nchunk, part, B,
                      R, rmax, active);
   NPPH_HIP(hipGetLastError());
