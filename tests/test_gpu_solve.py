@@ -114,6 +114,31 @@ def test_large_blocked_2048(hip):
     assert np.abs(x[0] - xo).max() <= 1e-13 * np.linalg.cond(a[0], 1) * np.abs(xo).max()
 
 
+def test_grouped_stack_1024(hip):
+    # 16 FP64 systems of order 1024: the factorisation runs as four concurrent
+    # groups of four matrices on their own streams. Pivots and factors of one
+    # matrix per group equal the oracle's; every system's backward error is
+    # FP64's, and the FP32-refined solve of the same stack reaches 1e-12.
+    rng = np.random.default_rng(23)
+    count, n = 16, 1024
+    a = rng.uniform(-1, 1, (count, n, n))
+    b = rng.standard_normal((count, n, 3))
+    lu, x, ipiv, info = gesv_host(a, b)
+    assert np.all(info == 0)
+    for i in range(count):
+        assert np.all(oracle.backward_error(a[i], b[i], x[i]) < 1e-14), i
+    for i in (1, 6, 11, 12):
+        xo, luo, steps = oracle.solve(a[i], b[i], with_factors=True)
+        if pivot_gaps(a[i]) > 1e-6:
+            np.testing.assert_array_equal(ipiv[i], steps)
+            assert np.abs(lu[i] - luo).max() <= 1e-10 * np.abs(luo).max(), i
+        assert np.abs(x[i] - xo).max() <= 1e-13 * np.linalg.cond(a[i], 1) * np.abs(xo).max(), i
+    xr, berr, iters, rinfo = nla.solve_refined(a, b, precision="fp32")
+    assert np.all(rinfo == 0) and np.all(berr <= 1e-12)
+    for i in (0, 5, 10, 15):
+        assert np.all(oracle.backward_error(a[i], b[i], xr[i]) <= 1e-12), i
+
+
 def test_large_blocked_pivots_2100(hip):
     # n = 2100: the first panels span more than 2048 rows and the last one is
     # 52 columns wide (partial inner blocks); pivots must equal the oracle's.
