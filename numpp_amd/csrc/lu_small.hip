@@ -4,8 +4,8 @@
 // The matrix is padded to N = 8, 16, 32 or 64 with the identity (which leaves
 // the leading n x n factorisation, its pivots and its exact-zero singularity
 // test unchanged) and split by columns over NW = N/16 waves (one wave for
-// N <= 16): lane r of every wave holds original row r, wave w the columns
-// [w CPW, (w+1) CPW). Rows are never moved: each wave tracks the logical
+// N <= 16; eight waves for N = 64 with one right-hand side): lane r of every
+// wave holds original row r, wave w the columns [w CPW, (w+1) CPW). Rows are never moved: each wave tracks the logical
 // position `pos` of every lane's row, and the step-k swap of positions k and p
 // (linalg.cpp:58-62) exchanges two lanes' `pos` values.
 //
@@ -26,6 +26,7 @@
 // solve reads them: forward substitution with the unit-lower L, then back
 // substitution with U (linalg.cpp:73-90), lane = logical row, waves taking
 // right-hand-side chunks in turn.
+#include <cstdlib>
 #include <utility>
 
 #include "common.hpp"
@@ -37,9 +38,9 @@ namespace npph {
 namespace {
 
 
-template <int N>
+template <int N, int WV = (N >= 32 ? N / 16 : 1)>
 struct Small {
-  static constexpr int NW = N >= 32 ? N / 16 : 1;  // waves per matrix
+  static constexpr int NW = WV;                    // waves per matrix
   static constexpr int CPW = N / NW;               // columns per wave
   static constexpr int NT = 64 * NW;
 };
@@ -63,10 +64,10 @@ struct SmallShared {
 // Factorisation, as seen by wave W (all arguments uniform across the wave).
 // AUG: the last wave also carries b as an extra column (ab, by physical row),
 // so the forward substitution happens inside the elimination.
-template <int N, int W, bool AUG>
-__device__ __forceinline__ void factor(SmallShared<N>& sh, double (&a)[Small<N>::CPW], int& pos, int n, int lane,
+template <int N, int WV, int W, bool AUG>
+__device__ __forceinline__ void factor(SmallShared<N>& sh, double (&a)[Small<N, WV>::CPW], int& pos, int n, int lane,
                                        double& ab) {
-  constexpr int CPW = Small<N>::CPW;
+  constexpr int CPW = Small<N, WV>::CPW;
   bool stop = false;
   auto step = [&]<int K>() __attribute__((always_inline)) {
     if (stop || K >= n) return;  // uniform over the workgroup (every wave stops at the same step)
@@ -119,7 +120,7 @@ __device__ __forceinline__ void factor(SmallShared<N>& sh, double (&a)[Small<N>:
         a[c] = fma(-l, u, a[c]);
       }
     }
-    if constexpr (AUG && W == Small<N>::NW - 1) ab = fma(-l, readlane_d(ab, wl), ab);
+    if constexpr (AUG && W == Small<N, WV>::NW - 1) ab = fma(-l, readlane_d(ab, wl), ab);
   };
   [&]<int... K>(std::integer_sequence<int, K...>) __attribute__((always_inline)) {
     (step.template operator()<K>(), ...);
@@ -129,9 +130,9 @@ __device__ __forceinline__ void factor(SmallShared<N>& sh, double (&a)[Small<N>:
 // NR: right-hand-side columns held in registers per pass (1 for a single
 // right-hand side, which then rides along the elimination as an extra column,
 // else 8).
-template <int N, int W, int NR>
+template <int N, int WV, int W, int NR>
 __device__ __forceinline__ void wave_body(const UniformBatch& u, SmallShared<N>& sh, int64_t mat, int lane) {
-  constexpr int CPW = Small<N>::CPW, NW = Small<N>::NW;
+  constexpr int CPW = Small<N, WV>::CPW, NW = Small<N, WV>::NW;
   const int n = u.n;
   double a[CPW];
   {
@@ -147,7 +148,7 @@ __device__ __forceinline__ void wave_body(const UniformBatch& u, SmallShared<N>&
   double ab = 0.0;
   if (AUG && W == NW - 1 && u.nrhs > 0 && u.X && lane < n) ab = u.B[mat * u.strideB + lane];
   int pos = lane < N ? lane : (1 << 20);
-  factor<N, W, AUG>(sh, a, pos, n, lane, ab);
+  factor<N, WV, W, AUG>(sh, a, pos, n, lane, ab);
   const bool singular = sh.sing >= 0;
   if (W == 0 && lane == 0 && u.info) u.info[mat] = singular ? sh.sing + 1 : 0;
   if (W == 0 && u.ipiv) {
@@ -245,8 +246,8 @@ __device__ __forceinline__ void wave_body(const UniformBatch& u, SmallShared<N>&
 }
 
 // Uniform stack: one workgroup of NW waves per matrix.
-template <int N, int NR>
-__global__ __launch_bounds__(Small<N>::NT) void lu_small_kernel(UniformBatch u) {
+template <int N, int NR, int WV>
+__global__ __launch_bounds__((Small<N, WV>::NT)) void lu_small_kernel(UniformBatch u) {
   __shared__ SmallShared<N> sh;
   const int64_t mat = blockIdx.x;
   // wave index made provably uniform, so the per-wave bodies are scalar
@@ -255,8 +256,8 @@ __global__ __launch_bounds__(Small<N>::NT) void lu_small_kernel(UniformBatch u) 
   if (threadIdx.x == 0) sh.sing = -1;
   __syncthreads();
   [&]<int... W>(std::integer_sequence<int, W...>) __attribute__((always_inline)) {
-    ((wave == W ? (wave_body<N, W, NR>(u, sh, mat, lane), 0) : 0), ...);
-  }(std::make_integer_sequence<int, Small<N>::NW>{});
+    ((wave == W ? (wave_body<N, WV, W, NR>(u, sh, mat, lane), 0) : 0), ...);
+  }(std::make_integer_sequence<int, Small<N, WV>::NW>{});
 }
 
 }  // namespace
@@ -274,9 +275,29 @@ int small_lu_launch(const UniformBatch& u, hipStream_t st) {
                8.0 * u.count * (n * n + 2 * n * u.nrhs + (u.LU ? n * n : 0)), st);
   // a single right-hand side (or none) solves one register column per pass
   const bool one = u.nrhs <= 1;
+  // order 64 with one right-hand side: the columns split over eight waves (the
+  // per-step update of each wave's columns, which every step waits for, is
+  // half of what four waves do; 256 x 64^2: 51.6 -> 49.3 us per solve with
+  // the input copies); NUMPP_HIP_SMALL_WAVES=4 or 16 selects another split
+  static const int waves64 = [] {
+    const char* e = std::getenv("NUMPP_HIP_SMALL_WAVES");
+    const int v = e ? std::atoi(e) : 8;
+    return v == 4 || v == 16 ? v : 8;
+  }();
   auto launch = [&]<int NV>() {
-    if (one) hipLaunchKernelGGL((lu_small_kernel<NV, 1>), grid, dim3(Small<NV>::NT), 0, st, u);
-    else hipLaunchKernelGGL((lu_small_kernel<NV, 8>), grid, dim3(Small<NV>::NT), 0, st, u);
+    constexpr int W0 = Small<NV>::NW;
+    if constexpr (NV == 64) {
+      if (one && waves64 == 8) {
+        hipLaunchKernelGGL((lu_small_kernel<NV, 1, 8>), grid, dim3(Small<NV, 8>::NT), 0, st, u);
+        return;
+      }
+      if (one && waves64 == 16) {
+        hipLaunchKernelGGL((lu_small_kernel<NV, 1, 16>), grid, dim3(Small<NV, 16>::NT), 0, st, u);
+        return;
+      }
+    }
+    if (one) hipLaunchKernelGGL((lu_small_kernel<NV, 1, W0>), grid, dim3(Small<NV>::NT), 0, st, u);
+    else hipLaunchKernelGGL((lu_small_kernel<NV, 8, W0>), grid, dim3(Small<NV>::NT), 0, st, u);
   };
   if (u.n <= 8) launch.template operator()<8>();
   else if (u.n <= 16) launch.template operator()<16>();
