@@ -341,7 +341,10 @@ int getrf_group(int64_t count, int n, T* A, int64_t strideA, int32_t* ipiv, int3
 // mutex covers only the issue, not the execution).
 constexpr int kMaxGroups = 4;
 struct GroupStreams {
-  std::mutex mu;
+  // held for an issue, and for the whole capture of a grouped factorisation
+  // (its branch streams belong to that capture until it ends, so no other
+  // thread may queue work on them meanwhile)
+  std::recursive_mutex mu;
   hipStream_t st[kMaxGroups] = {};
   hipEvent_t fork = nullptr, join[kMaxGroups] = {};
 };
@@ -390,7 +393,7 @@ int getrf_grouped(int G, int64_t count, int n, T* A, int64_t strideA, int32_t* i
                   const int32_t* mask, hipStream_t s) {
   GroupStreams* gs = nullptr;
   NPPH_TRY(group_streams(&gs));
-  std::lock_guard<std::mutex> lk(gs->mu);
+  std::lock_guard<std::recursive_mutex> lk(gs->mu);
   NPPH_HIP(hipEventRecord(gs->fork, s));
   for (int g = 0; g < G; ++g) NPPH_HIP(hipStreamWaitEvent(gs->st[g], gs->fork, 0));
   int st = NPPH_OK;
@@ -514,10 +517,17 @@ int blocked_getrf_perm(int64_t count, int n, T* A, int64_t strideA, int32_t* ipi
     }
     return st;
   }
+  std::unique_lock<std::recursive_mutex> glk;
+  if (G > 1) {
+    GroupStreams* gs = nullptr;
+    NPPH_TRY(group_streams(&gs));
+    glk = std::unique_lock<std::recursive_mutex>(gs->mu);
+  }
   NPPH_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
   const int st = issue();
   hipGraph_t graph = nullptr;
   const hipError_t ec = hipStreamEndCapture(s, &graph);
+  if (glk.owns_lock()) glk.unlock();
   if (st != NPPH_OK || ec != hipSuccess || workspace_generation() != key.gen) {
     if (graph) (void)hipGraphDestroy(graph);
     if (st != NPPH_OK) return st;
